@@ -1,0 +1,388 @@
+// Streaming HIP kernels for gfx950: in-place k-bit operators on a state
+// vector (or on vec(rho)), FP64 reductions, probabilities and helpers.
+//
+// Every operator pass is one read and one write of the 2^K amplitudes a
+// thread owns (64-wide wavefronts, consecutive lanes on consecutive groups so
+// that the free low bits give coalesced addresses); reductions accumulate in
+// FP64 and finish in a fixed order, so they are bit-reproducible.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "bt_internal.hpp"
+
+namespace bt {
+
+template <typename R, int K>
+struct ApplyArgs {
+  static constexpr int D = 1 << K;
+  static constexpr int kInline = (K <= 3) ? D * D : 1;
+  C<R>* s;
+  uint64_t ngroups;
+  uint64_t ctrl;
+  uint64_t off[D];
+  int nfixed;
+  int fixed[kMaxFixed];
+  R scale;
+  const C<R>* mptr;  // K >= 4: device copy of the matrix (column-major)
+  C<R> m[kInline];   // K <= 3: matrix inline in the kernel arguments
+};
+
+template <typename R, int K>
+__global__ void __launch_bounds__(256) k_apply(ApplyArgs<R, K> a) {
+  constexpr int D = 1 << K;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.ngroups; g += stride) {
+    uint64_t base = g;
+    for (int i = 0; i < a.nfixed; i++) base = insert0(base, a.fixed[i]);
+    base |= a.ctrl;
+    C<R> v[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) v[l] = a.s[base + a.off[l]];
+#pragma unroll
+    for (int r = 0; r < D; r++) {
+      C<R> acc = cmake<C<R>>(0, 0);
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        const C<R> mrc = (K <= 3) ? a.m[r + c * D] : a.mptr[r + c * D];
+        acc = cfma(acc, mrc, v[c]);
+      }
+      acc.x *= a.scale;
+      acc.y *= a.scale;
+      a.s[base + a.off[r]] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-level reduction of M per-thread values; thread j < M writes the
+// block sum of slot j to out[j].
+template <int M>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[M], double* out) {
+  __shared__ double red[4][M];  // blockDim.x == 256 -> 4 waves
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    double v = wave_sum(acc[j]);
+    if (lane == 0) red[wave][j] = v;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < M; j += blockDim.x) out[j] = red[0][j] + red[1][j] + red[2][j] + red[3][j];
+}
+
+template <typename R, int K>
+struct ReduceArgs {
+  static constexpr int D = 1 << K;
+  const C<R>* s;
+  uint64_t ngroups;
+  uint64_t off[D];
+  int nfixed;
+  int fixed[kMaxFixed];
+  double* part;
+};
+
+// Packed layout of a reduced rho (D*D reals): [0, D) diagonal, then for
+// each a < b (row-major over a): Re, Im of rho[a, b] = sum v_a conj(v_b).
+template <typename R, int K>
+__global__ void __launch_bounds__(256) k_reduce(ReduceArgs<R, K> a) {
+  constexpr int D = 1 << K;
+  constexpr int M = D * D;
+  double acc[M];
+#pragma unroll
+  for (int j = 0; j < M; j++) acc[j] = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.ngroups; g += stride) {
+    uint64_t base = g;
+    for (int i = 0; i < a.nfixed; i++) base = insert0(base, a.fixed[i]);
+    double vr[D], vi[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+      C<R> v = a.s[base + a.off[l]];
+      vr[l] = (double)v.x;
+      vi[l] = (double)v.y;
+    }
+    int p = D;
+#pragma unroll
+    for (int x = 0; x < D; x++) {
+      acc[x] = fma(vr[x], vr[x], fma(vi[x], vi[x], acc[x]));
+#pragma unroll
+      for (int y = x + 1; y < D; y++) {
+        // v_x conj(v_y)
+        acc[p] += vr[x] * vr[y] + vi[x] * vi[y];
+        acc[p + 1] += vi[x] * vr[y] - vr[x] * vi[y];
+        p += 2;
+      }
+    }
+  }
+  block_reduce_store<M>(acc, a.part + (size_t)blockIdx.x * M);
+}
+
+__global__ void k_finalize(const double* part, int nblocks, int M, double* out) {
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    double t = 0.0;
+    for (int b = 0; b < nblocks; b++) t += part[(size_t)b * M + j];
+    out[j] = t;
+  }
+}
+
+// sum over i of Re(s[i * step]) for i < count (trace of vec(rho): step = D+1)
+template <typename R>
+__global__ void __launch_bounds__(256) k_strided_re_sum(const C<R>* s, uint64_t count, uint64_t step, double* part) {
+  double acc[1] = {0.0};
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+    acc[0] += (double)s[i * step].x;
+  block_reduce_store<1>(acc, part + blockIdx.x);
+}
+
+template <typename R>
+__global__ void k_probabilities(const C<R>* s, uint64_t count, uint64_t step, double* out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    C<R> v = s[i * step];
+    out[i] = step == 1 ? (double)v.x * (double)v.x + (double)v.y * (double)v.y : (double)v.x;
+  }
+}
+
+template <typename R>
+__global__ void k_scale(C<R>* s, uint64_t n, R f) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    C<R> v = s[i];
+    v.x *= f;
+    v.y *= f;
+    s[i] = v;
+  }
+}
+
+template <typename R>
+__global__ void k_fill_basis(C<R>* s, uint64_t n, uint64_t index) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    s[i] = cmake<C<R>>(i == index ? (R)1 : (R)0, (R)0);
+}
+
+template <typename R>
+__global__ void k_gather(const C<R>* s, uint64_t len, const uint64_t* idx, uint64_t n, double* out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t j = idx[i];
+    C<R> v = j < len ? s[j] : cmake<C<R>>(0, 0);
+    out[2 * i] = (double)v.x;
+    out[2 * i + 1] = (double)v.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static unsigned grid_for(uint64_t work, unsigned block = 256, uint64_t cap = (1ull << 20)) {
+  uint64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+int make_placement(int nbits, int k, const int* bits, uint64_t ctrl_mask, Placement* p) {
+  if (k < 0 || k > kMaxK) return fail(BT_ERR_ARG, "operator width k out of range");
+  uint64_t used = 0;
+  for (int i = 0; i < k; i++) {
+    if (bits[i] < 0 || bits[i] >= nbits) return fail(BT_ERR_ARG, "bit position out of range");
+    if (used & (1ull << bits[i])) return fail(BT_ERR_ARG, "repeated bit position");
+    used |= 1ull << bits[i];
+  }
+  if (nbits < 64 && (ctrl_mask >> nbits)) return fail(BT_ERR_ARG, "control bit out of range");
+  if (ctrl_mask & used) return fail(BT_ERR_ARG, "control bit equals a target bit");
+  uint64_t all = used | ctrl_mask;
+  p->nfixed = 0;
+  for (int b = 0; b < nbits; b++)
+    if (all & (1ull << b)) {
+      if (p->nfixed >= kMaxFixed) return fail(BT_ERR_ARG, "too many target/control bits");
+      p->fixed[p->nfixed++] = b;
+    }
+  p->ctrl = ctrl_mask;
+  for (int l = 0; l < (1 << k); l++) {
+    uint64_t o = 0;
+    for (int i = 0; i < k; i++)
+      if ((l >> (k - 1 - i)) & 1) o |= 1ull << bits[i];
+    p->off[l] = o;
+  }
+  return BT_OK;
+}
+
+template <typename R, int K>
+static int apply_t(bt_state_s* s, int nbits, const double* m, const Placement& p, double scale, bool conj_mat) {
+  constexpr int D = 1 << K;
+  ApplyArgs<R, K> a;
+  memset(&a, 0, sizeof a);
+  a.s = (C<R>*)s->d;
+  a.ngroups = 1ull << (nbits - p.nfixed);
+  a.ctrl = p.ctrl;
+  for (int l = 0; l < D; l++) a.off[l] = p.off[l];
+  a.nfixed = p.nfixed;
+  for (int i = 0; i < p.nfixed; i++) a.fixed[i] = p.fixed[i];
+  a.scale = (R)scale;
+  const double sgn = conj_mat ? -1.0 : 1.0;
+  if (K <= 3) {
+    for (int i = 0; i < D * D; i++) a.m[i] = cmake<C<R>>((R)m[2 * i], (R)(sgn * m[2 * i + 1]));
+    a.mptr = nullptr;
+  } else {
+    std::vector<C<R>> hm(D * D);
+    for (int i = 0; i < D * D; i++) hm[i] = cmake<C<R>>((R)m[2 * i], (R)(sgn * m[2 * i + 1]));
+    if (!s->d_mat) BT_HIP(hipMalloc(&s->d_mat, sizeof(C<double>) * 1024));
+    // the matrix must not change while an earlier launch may still read it
+    BT_HIP(hipStreamSynchronize(s->stream));
+    BT_HIP(hipMemcpy(s->d_mat, hm.data(), sizeof(C<R>) * D * D, hipMemcpyHostToDevice));
+    a.mptr = (const C<R>*)s->d_mat;
+  }
+  hipLaunchKernelGGL((k_apply<R, K>), dim3(grid_for(a.ngroups)), dim3(256), 0, s->stream, a);
+  BT_HIP(hipGetLastError());
+  return BT_OK;
+}
+
+int launch_apply(bt_state_s* s, int nbits, int k, const double* m, const Placement& p, double scale,
+                 bool conj_mat) {
+  if (nbits - p.nfixed < 0) return fail(BT_ERR_ARG, "placement wider than the state");
+  const bool f = s->prec == BT_C64;
+  switch (k) {
+    case 1: return f ? apply_t<float, 1>(s, nbits, m, p, scale, conj_mat) : apply_t<double, 1>(s, nbits, m, p, scale, conj_mat);
+    case 2: return f ? apply_t<float, 2>(s, nbits, m, p, scale, conj_mat) : apply_t<double, 2>(s, nbits, m, p, scale, conj_mat);
+    case 3: return f ? apply_t<float, 3>(s, nbits, m, p, scale, conj_mat) : apply_t<double, 3>(s, nbits, m, p, scale, conj_mat);
+    case 4: return f ? apply_t<float, 4>(s, nbits, m, p, scale, conj_mat) : apply_t<double, 4>(s, nbits, m, p, scale, conj_mat);
+    case 5: return f ? apply_t<float, 5>(s, nbits, m, p, scale, conj_mat) : apply_t<double, 5>(s, nbits, m, p, scale, conj_mat);
+    default: return fail(BT_ERR_ARG, "operator width k out of range");
+  }
+}
+
+template <typename R, int K>
+static int reduce_t(bt_state_s* s, const Placement& p) {
+  constexpr int D = 1 << K;
+  constexpr int M = D * D;
+  ReduceArgs<R, K> a;
+  memset(&a, 0, sizeof a);
+  a.s = (const C<R>*)s->d;
+  a.ngroups = 1ull << (s->nbits - p.nfixed);
+  for (int l = 0; l < D; l++) a.off[l] = p.off[l];
+  a.nfixed = p.nfixed;
+  for (int i = 0; i < p.nfixed; i++) a.fixed[i] = p.fixed[i];
+  a.part = s->d_part;
+  const unsigned blocks = grid_for(a.ngroups, 256, kRedBlocks);
+  hipLaunchKernelGGL((k_reduce<R, K>), dim3(blocks), dim3(256), 0, s->stream, a);
+  BT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s->stream, (const double*)s->d_part, (int)blocks, M, s->d_out);
+  BT_HIP(hipGetLastError());
+  BT_HIP(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * M, hipMemcpyDeviceToHost, s->stream));
+  BT_HIP(hipStreamSynchronize(s->stream));
+  return BT_OK;
+}
+
+int launch_reduce(bt_state_s* s, int k, const int* bits) {
+  Placement p;
+  int rc = make_placement(s->nbits, k, bits, 0, &p);
+  if (rc) return rc;
+  const bool f = s->prec == BT_C64;
+  switch (k) {
+    case 0: return f ? reduce_t<float, 0>(s, p) : reduce_t<double, 0>(s, p);
+    case 1: return f ? reduce_t<float, 1>(s, p) : reduce_t<double, 1>(s, p);
+    case 2: return f ? reduce_t<float, 2>(s, p) : reduce_t<double, 2>(s, p);
+    case 3: return f ? reduce_t<float, 3>(s, p) : reduce_t<double, 3>(s, p);
+    default: return fail(BT_ERR_ARG, "reduction width k out of range");
+  }
+}
+
+void unpack_reduced(int k, const double* packed, double* rho) {
+  const int D = 1 << k;
+  for (int i = 0; i < 2 * D * D; i++) rho[i] = 0.0;
+  for (int a = 0; a < D; a++) rho[2 * (a + a * D)] = packed[a];
+  int p = D;
+  for (int a = 0; a < D; a++)
+    for (int b = a + 1; b < D; b++) {
+      // rho[a, b] = re + i im ; rho[b, a] = conj
+      rho[2 * (a + b * D)] = packed[p];
+      rho[2 * (a + b * D) + 1] = packed[p + 1];
+      rho[2 * (b + a * D)] = packed[p];
+      rho[2 * (b + a * D) + 1] = -packed[p + 1];
+      p += 2;
+    }
+}
+
+int launch_norm2(bt_state_s* s, double* out) {
+  int rc = launch_reduce(s, 0, nullptr);
+  if (rc) return rc;
+  *out = s->h_out[0];
+  return BT_OK;
+}
+
+int launch_trace(bt_state_s* s, double* out) {
+  const uint64_t D = 1ull << s->n;
+  const unsigned blocks = grid_for(D, 256, kRedBlocks);
+  if (s->prec == BT_C64)
+    hipLaunchKernelGGL(k_strided_re_sum<float>, dim3(blocks), dim3(256), 0, s->stream, (const C<float>*)s->d, D, D + 1,
+                       s->d_part);
+  else
+    hipLaunchKernelGGL(k_strided_re_sum<double>, dim3(blocks), dim3(256), 0, s->stream, (const C<double>*)s->d, D,
+                       D + 1, s->d_part);
+  BT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s->stream, (const double*)s->d_part, (int)blocks, 1, s->d_out);
+  BT_HIP(hipGetLastError());
+  BT_HIP(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  BT_HIP(hipStreamSynchronize(s->stream));
+  *out = s->h_out[0];
+  return BT_OK;
+}
+
+int launch_probabilities(bt_state_s* s, double* d_out) {
+  const uint64_t D = 1ull << s->n;
+  const uint64_t step = s->kind == BT_DM ? D + 1 : 1;
+  if (s->prec == BT_C64)
+    hipLaunchKernelGGL(k_probabilities<float>, dim3(grid_for(D)), dim3(256), 0, s->stream, (const C<float>*)s->d, D,
+                       step, d_out);
+  else
+    hipLaunchKernelGGL(k_probabilities<double>, dim3(grid_for(D)), dim3(256), 0, s->stream, (const C<double>*)s->d, D,
+                       step, d_out);
+  BT_HIP(hipGetLastError());
+  return BT_OK;
+}
+
+int launch_scale(bt_state_s* s, double f) {
+  if (s->prec == BT_C64)
+    hipLaunchKernelGGL(k_scale<float>, dim3(grid_for(s->len)), dim3(256), 0, s->stream, (C<float>*)s->d, s->len,
+                       (float)f);
+  else
+    hipLaunchKernelGGL(k_scale<double>, dim3(grid_for(s->len)), dim3(256), 0, s->stream, (C<double>*)s->d, s->len, f);
+  BT_HIP(hipGetLastError());
+  return BT_OK;
+}
+
+int launch_set_basis(bt_state_s* s, uint64_t index) {
+  if (s->prec == BT_C64)
+    hipLaunchKernelGGL(k_fill_basis<float>, dim3(grid_for(s->len)), dim3(256), 0, s->stream, (C<float>*)s->d, s->len,
+                       index);
+  else
+    hipLaunchKernelGGL(k_fill_basis<double>, dim3(grid_for(s->len)), dim3(256), 0, s->stream, (C<double>*)s->d,
+                       s->len, index);
+  BT_HIP(hipGetLastError());
+  return BT_OK;
+}
+
+int launch_gather(bt_state_s* s, const uint64_t* d_idx, uint64_t n, double* d_out) {
+  if (s->prec == BT_C64)
+    hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, s->stream, (const C<float>*)s->d, s->len,
+                       d_idx, n, d_out);
+  else
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, s->stream, (const C<double>*)s->d, s->len,
+                       d_idx, n, d_out);
+  BT_HIP(hipGetLastError());
+  return BT_OK;
+}
+
+}  // namespace bt
