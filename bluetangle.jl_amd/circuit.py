@@ -1,0 +1,269 @@
+"""Circuit IR, and the circuit drivers run / to_state / to_rho (src/ops.jl:659-844).
+
+``compile_ir`` lowers an op list plus a NoiseModel into the engine's flat IR
+(bt_ir_op), expanding the noise routing of apply_noise (hilbert.jl:322-340)
+into explicit channel ops right after each noisy gate, so that the engine
+consumes uniforms in the reference's order.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import List, Optional, Sequence, Union
+
+import numpy as np
+
+from . import _lib as L
+from .core import (DensityMatrix, StateVector, _colmajor, channel_bits, init, zero_rho, zero_state)
+from .gates import MEASUREMENTS, gate
+from .noise import NoiseModel, QuantumChannel
+from .ops import Op, OpF, OpQC, QuantumOps, ifOp
+
+
+class IR:
+    """Flat op array + matrix pool (complex128, column-major)."""
+
+    def __init__(self, nqubits: int):
+        self.n = nqubits
+        self.ops: List[L.IrOp] = []
+        self.pool: List[np.ndarray] = []
+        self.pool_len = 0
+
+    def add_matrix(self, m) -> int:
+        m = np.asarray(m, dtype=np.complex128)
+        off = self.pool_len
+        self.pool.append(_colmajor(m))
+        self.pool_len += m.shape[0] * m.shape[1]
+        return off
+
+    def op(self, kind, bits, k=None, prob_bits=None, ctrl=0, mat=0, nkraus=0, basis=0, record=0):
+        o = L.IrOp()
+        o.kind = kind
+        o.k = len(bits) if k is None else k
+        for i, b in enumerate(bits):
+            o.bits[i] = b
+        for i, b in enumerate(prob_bits or []):
+            o.prob_bits[i] = b
+        o.ctrl_mask = ctrl
+        o.mat = mat
+        o.nkraus = nkraus
+        o.basis = basis
+        o.record = record
+        return o
+
+    def arrays(self):
+        arr = (L.IrOp * max(1, len(self.ops)))(*self.ops)
+        pool = np.concatenate(self.pool) if self.pool else np.zeros(2)
+        return arr, np.ascontiguousarray(pool)
+
+
+def _bit(N, q):
+    if q < 1 or q > N:
+        raise ValueError("N must be larger than qubit")
+    return N - q
+
+
+def _gate_record(ir, N, q, mat, qubit, target, control):
+    if N < qubit or N < target or N < control:
+        raise ValueError("N must be larger than qubits")
+    ctrl = (1 << _bit(N, control)) if control != -2 else 0
+    if q == 1:
+        bits = [_bit(N, qubit)]
+    else:
+        bits = [_bit(N, qubit), _bit(N, target)]
+        if control != -2 and abs(qubit - target) != 1 and not (
+                np.array_equal(mat, gate["CX"]) or np.array_equal(mat, gate["CZ"])):
+            raise ValueError("Unsupported operation. Use 'CCZ' or 'CCX'.")
+    return ir.op(L.BT_OP_GATE, bits, ctrl=ctrl, mat=ir.add_matrix(mat))
+
+
+def _channel_record(ir, N, kraus, qubit, target):
+    q = int(round(math.log2(kraus[0].shape[0])))
+    a, p = channel_bits(N, q, qubit, target)
+    off = ir.add_matrix(kraus[0])
+    for K in kraus[1:]:
+        ir.add_matrix(K)
+    return ir.op(L.BT_OP_CHANNEL, a, prob_bits=p, mat=off, nkraus=len(kraus))
+
+
+def _noise_records(ir, N, op, noise):
+    """apply_noise routing (hilbert.jl:322-340)."""
+    if not isinstance(noise, NoiseModel) or not getattr(op, "noisy", False):
+        return []
+
+    def ch(c, qubit, target):
+        if not isinstance(c, QuantumChannel):
+            raise ValueError("noise model has no channel for a %d-qubit operation" % op.q)
+        return _channel_record(ir, N, c.kraus, qubit, target)
+
+    if op.q == 1:
+        return [ch(noise.q1, op.qubit, -1)] if op.control == -2 else [ch(noise.q2, op.control, op.qubit)]
+    return [ch(noise.q2, op.qubit, op.target_qubit)]
+
+
+def _lower(ir, N, op, noise, pars, out: list, branches: list):
+    if isinstance(op, OpF):
+        if isinstance(op.data, (list, tuple)):
+            for o in op.data:  # o * state: apply without noise
+                _lower(ir, N, o, False, pars, out, branches)
+            return
+        raise TypeError("OpF with a function or full-space matrix cannot be lowered to the engine IR")
+    if isinstance(op, OpQC):
+        if op.name.upper() in ("RES", "RESET"):
+            out.append(ir.op(L.BT_OP_RESET, [_bit(N, op.qubit)], basis=L.BT_BASIS_Z))
+        else:
+            out.append(_channel_record(ir, N, op.kraus, op.qubit, op.target_qubit))
+        return
+    if op.type == "🔬":
+        basis = MEASUREMENTS[op.name.upper()]
+        if isinstance(op, ifOp):
+            rec = ir.op(L.BT_OP_IF, [_bit(N, op.qubit)], basis=basis, record=1)
+            b0, b1 = [], []
+            for o in op.if01[0]:
+                _lower(ir, N, o, noise, pars, b0, branches)
+            for o in op.if01[1]:
+                _lower(ir, N, o, noise, pars, b1, branches)
+            branches.append((rec, b0, b1))
+            out.append(rec)
+        else:
+            out.append(ir.op(L.BT_OP_MEASURE, [_bit(N, op.qubit)], basis=basis, record=1))
+        return
+    if op.type == "f":
+        if pars is None:
+            raise ValueError("function-type gates need pars=")
+        mat = np.asarray(op.mat(*pars), dtype=np.complex128)
+    else:
+        mat = op.mat
+    out.append(_gate_record(ir, N, op.q, mat, op.qubit, op.target_qubit, op.control))
+    out.extend(_noise_records(ir, N, op, noise))
+
+
+def compile_ir(ops: Sequence[QuantumOps], N: int, noise=False, pars=None) -> IR:
+    ir = IR(N)
+    top: List[L.IrOp] = []
+    branches = []
+    for op in ops:
+        _lower(ir, N, op, noise, pars, top, branches)
+    ir.ops = list(top)
+    for rec, b0, b1 in branches:
+        rec.br0_begin = len(ir.ops)
+        ir.ops.extend(b0)
+        rec.br0_end = len(ir.ops)
+        rec.br1_begin = len(ir.ops)
+        ir.ops.extend(b1)
+        rec.br1_end = len(ir.ops)
+    ir.ntop = len(top)
+    return ir
+
+
+def get_N(ops: Sequence[QuantumOps]) -> int:
+    """get_N(ops) (ops.jl:14-34)."""
+    m = 0
+    for op in ops:
+        if isinstance(op, OpF):
+            continue
+        if isinstance(op, ifOp):
+            m = max(m, op.qubit)
+        elif isinstance(op, OpQC):
+            m = max(m, op.qubit, op.target_qubit)
+        else:
+            m = max(m, op.qubit, op.target_qubit, op.control)
+    return m
+
+
+class Circuit:
+    """A compiled op list on the device (bt_circuit) with its options.
+    Stands in for the reference's Circuit + Options (struct.jl:835-873) on
+    the apply path: the noise model is folded into the IR."""
+
+    def __init__(self, ops: Sequence[QuantumOps], N: Optional[int] = None, noise=False, pars=None):
+        init()
+        self.N = get_N(ops) if N is None else N
+        self.noise = noise
+        self.ir = compile_ir(ops, self.N, noise, pars)
+        self.nops = len(self.ir.ops)
+        self.ntop = self.ir.ntop
+        self._arr, self._pool = self.ir.arrays()
+        h = ctypes.c_void_p()
+        L.check(L.lib().bt_circuit_create(self.N, self._arr, self.nops, self.ntop, L.dptr(self._pool),
+                                          self._pool.size // 2, ctypes.byref(h)))
+        self.handle = h
+        self.max_draws = int(L.lib().bt_circuit_max_draws(h))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and L._lib is not None:
+            try:
+                L._lib.bt_circuit_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    # --- drivers ---------------------------------------------------------
+    def run_on(self, state: StateVector, seed=0, traj=0, uniforms=None, outcap=None):
+        """One trajectory on `state` (in place).  Returns (outcomes, stats)."""
+        outcap = outcap or max(16, 2 * self.nops)
+        outs = np.zeros(outcap, dtype=np.int32)
+        nout = ctypes.c_int64()
+        st = L.RunStats()
+        if uniforms is not None:
+            u = np.ascontiguousarray(uniforms, dtype=np.float64)
+            up, nu = L.dptr(u), u.size
+        else:
+            up, nu = None, 0
+        L.check(L.lib().bt_run_circuit(state.handle, self.handle, seed, traj, up, nu, L.iptr(outs), outcap,
+                                       ctypes.byref(nout), ctypes.byref(st)))
+        return outs[: nout.value].tolist(), st.as_dict()
+
+    def run_dm(self, rho: DensityMatrix):
+        st = L.RunStats()
+        L.check(L.lib().bt_run_circuit_dm(rho.handle, self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def run_batch(self, ntraj: int, seed=0, traj0=0, prec="c64", keep_states=False, outcap=None):
+        """ntraj trajectories from |0..0> in one persistent launch.
+        Returns (outcome lists, states or None, stats)."""
+        from .core import PRECISIONS
+        outcap = outcap or max(16, 2 * self.nops)
+        outs = np.zeros((ntraj, outcap), dtype=np.int32)
+        nouts = np.zeros(ntraj, dtype=np.int64)
+        states = np.zeros((ntraj, 2 ** self.N), dtype=np.complex128) if keep_states else None
+        st = L.RunStats()
+        L.check(L.lib().bt_run_batch(self.handle, PRECISIONS[prec], seed, traj0, ntraj, L.iptr(outs), outcap,
+                                     nouts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     L.dptr(states.view(np.float64)) if keep_states else None,
+                                     ctypes.byref(st)))
+        return [outs[t, : nouts[t]].tolist() for t in range(ntraj)], states, st.as_dict()
+
+
+def to_state(ops_or_circuit, N=None, noise=False, seed=0, traj=0, prec="c64"):
+    """to_state (ops.jl:790-795): one trajectory from |0..0>."""
+    c = ops_or_circuit if isinstance(ops_or_circuit, Circuit) else Circuit(ops_or_circuit, N, noise)
+    st = zero_state(c.N, prec)
+    c.run_on(st, seed, traj)
+    return st
+
+
+def to_rho(ops_or_circuit, N=None, noise=False, prec="c64"):
+    """to_rho (ops.jl:806-844): the exact density matrix."""
+    c = ops_or_circuit if isinstance(ops_or_circuit, Circuit) else Circuit(ops_or_circuit, N, noise)
+    rho = zero_rho(c.N, prec)
+    c.run_dm(rho)
+    return rho
+
+
+def run(ops_or_circuit, number_of_experiment: int = 1, N=None, noise=False, seed=0, prec="c64", batched=True):
+    """run (ops.jl:659-692): mid-circuit outcomes per shot.  Shot t uses the
+    uniform stream (seed, t).  Batched: one persistent launch for all shots."""
+    if number_of_experiment <= 0:
+        raise ValueError("`number_of_experiment` must be positive.")
+    c = ops_or_circuit if isinstance(ops_or_circuit, Circuit) else Circuit(ops_or_circuit, N, noise)
+    if batched:
+        outs, _, _ = c.run_batch(number_of_experiment, seed=seed, prec=prec)
+        return outs
+    res = []
+    for t in range(number_of_experiment):
+        st = zero_state(c.N, prec)
+        o, _ = c.run_on(st, seed, t)
+        res.append(o)
+    return res

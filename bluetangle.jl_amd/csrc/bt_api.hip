@@ -346,8 +346,11 @@ int bt_apply_kraus_traj(bt_state s, int k, int nkraus, const double* kraus, cons
   double tr = 0;
   rc = reduced(s, k, prob_bits, rho, &tr);
   if (rc) return rc;
+  // the reference samples from the state as it is (unnormalised after a
+  // non-unitary gate): p_i = Re tr(K_i rho_A K_i') without dividing by tr
+  (void)tr;
   std::vector<double> p(nkraus);
-  for (int i = 0; i < nkraus; i++) p[i] = sandwich_trace(kraus + (size_t)i * 2 * D * D, rho, D) / tr;
+  for (int i = 0; i < nkraus; i++) p[i] = sandwich_trace(kraus + (size_t)i * 2 * D * D, rho, D);
   int ind = weighted_pick(p, u);
   if (chosen) *chosen = ind;
   if (probs_out)
@@ -437,7 +440,8 @@ int bt_measure(bt_state s, int bit, int basis, const double* u, int* outcome, do
         for (int y = 0; y < 2; y++) v += R[a + 2 * x] * rho[x + 2 * y] * std::conj(R[b + 2 * y]);
       Rr[a + 2 * b] = v;
     }
-  const double p0 = Rr[0].real() / tr;
+  (void)tr;
+  const double p0 = Rr[0].real();  // prob0 of the state as it is (hilbert.jl:690)
   const int ind = ud < p0 ? 0 : 1;
   if (outcome) *outcome = ind;
   if (prob0) *prob0 = p0;
@@ -463,7 +467,8 @@ int bt_reset(bt_state s, int bit, double u, int* outcome) {
   double tr = 0;
   rc = reduced(s, 1, &bit, rho, &tr);
   if (rc) return rc;
-  const double p0 = rho[0].real() / tr;
+  (void)tr;
+  const double p0 = rho[0].real();
   const int ind = u < p0 ? 0 : 1;
   if (outcome) *outcome = ind;
   // P0 (outcome 0) or X P1 = |0><1| (outcome 1), renormalised

@@ -162,6 +162,11 @@ int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats);
 int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes, int64_t outcap,
                  int64_t* nouts, double* final_states, bt_run_stats* stats);
 
+/* Tile width (bits held in LDS per pass) of the circuit executors for a
+ * precision; 0 restores the default (14 for BT_C64, 13 for BT_C128).  Set by
+ * the autotuner.  Process-wide. */
+int bt_set_tile_bits(int prec, int tile_bits);
+
 /* ---- serving: continuous batching with HBM-budget admission ------------- */
 typedef struct {
   int64_t id;

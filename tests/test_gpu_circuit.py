@@ -1,0 +1,158 @@
+"""GPU parity of the circuit executors against the FP64 oracle:
+bt_run_circuit (one trajectory, grid tile passes), bt_run_batch (persistent
+batched trajectories), bt_run_circuit_dm (to_rho).
+
+Tolerances: complex128 1e-12, complex64 1e-5 (max amplitude / entry
+error); outcomes identical unless the oracle counts a draw within 1e-6 of a
+cumulative-probability boundary."""
+import numpy as np
+import pytest
+
+import oracle as O
+from refcirc import brickwork_spec, random_spec, to_bt, to_ref
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"c128": 1e-12, "c64": 1e-5}
+KINDS = ("gate1", "gate2", "ccx", "chan1", "chan2", "meas", "reset", "ifop")
+
+
+@pytest.fixture
+def tiles(bt):
+    from bluetangle_jl_amd import _lib
+    L = _lib.lib()
+
+    def set_tiles(T):
+        _lib.check(L.bt_set_tile_bits(0, T))
+        _lib.check(L.bt_set_tile_bits(1, T))
+    yield set_tiles
+    set_tiles(0)
+
+
+def oracle_traj(N, spec, nm_args, seed, traj):
+    nm = O.ref_noise(*nm_args) if nm_args else None
+    u = O.uniforms(seed, traj, 16 * len(spec) + 16)
+    s, outs, used, bnd, _ = O.run_sv(N, to_ref(spec), nm, u, tol=1e-6)
+    return s, outs, bnd
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("T", [0, 3])
+def test_run_circuit_random(bt, tiles, prec, T):
+    from bluetangle_jl_amd.circuit import Circuit
+    tiles(T)
+    for seed in range(6):
+        rng = np.random.default_rng(300 + seed)
+        N = 4 + seed % 4
+        spec = random_spec(N, 50, rng, kinds=KINDS)
+        nm_args = ("depolarizing", 0.05, "amplitude_damping", 0.1)
+        c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
+        st = bt.zero_state(N, prec)
+        outs, stats = c.run_on(st, seed=seed, traj=0)
+        r, routs, bnd = oracle_traj(N, spec, nm_args, seed, 0)
+        if bnd:
+            continue
+        s = st.to_numpy()
+        assert outs == routs, (seed, prec, T)
+        assert np.array_equal(np.isnan(s), np.isnan(r))
+        assert np.allclose(s, r, rtol=0, atol=TOL[prec], equal_nan=True), (seed, np.nanmax(np.abs(s - r)))
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("T", [0, 5])
+def test_run_batch_matches_per_trajectory_oracle(bt, tiles, prec, T):
+    from bluetangle_jl_amd.circuit import Circuit
+    tiles(T)
+    N = 8
+    spec = brickwork_spec(N, 6, 11, ad_gamma=5e-3, mz_prob=0.25)
+    nm_args = ("depolarizing", 1e-3, "depolarizing", 1e-2)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
+    ntraj = 48
+    outs, states, stats = c.run_batch(ntraj, seed=21, prec=prec, keep_states=True)
+    assert stats["passes"] > 0
+    skipped = 0
+    for t in range(ntraj):
+        r, routs, bnd = oracle_traj(N, spec, nm_args, 21, t)
+        if bnd:
+            skipped += 1
+            continue
+        assert outs[t] == routs, t
+        assert np.abs(states[t] - r).max() < TOL[prec], (t, np.abs(states[t] - r).max())
+    assert skipped < ntraj // 4
+
+
+def test_run_batch_random_ops(bt, tiles):
+    """Every op kind through the persistent kernel (c128)."""
+    from bluetangle_jl_amd.circuit import Circuit
+    tiles(3)
+    rng = np.random.default_rng(7)
+    N = 6
+    spec = random_spec(N, 60, rng, kinds=KINDS)
+    nm_args = ("depolarizing", 0.05, "amplitude_damping", 0.1)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
+    outs, states, _ = c.run_batch(16, seed=3, prec="c128", keep_states=True)
+    for t in range(16):
+        r, routs, bnd = oracle_traj(N, spec, nm_args, 3, t)
+        if bnd:
+            continue
+        assert outs[t] == routs
+        assert np.allclose(states[t], r, rtol=0, atol=1e-12, equal_nan=True)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_run_circuit_dm_random(bt, tiles, prec):
+    from bluetangle_jl_amd.circuit import Circuit
+    for T, seed in [(0, 1), (4, 2), (0, 3)]:
+        tiles(T)
+        rng = np.random.default_rng(400 + seed)
+        N = 3 + seed
+        spec = random_spec(N, 40, rng, kinds=("gate1", "gate2", "ccx", "chan1", "chan2", "reset"))
+        nm_args = ("depolarizing", 0.05, "amplitude_damping", 0.1)
+        c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
+        rho = bt.zero_rho(N, prec)
+        c.run_dm(rho)
+        ref = O.run_dm(N, to_ref(spec), O.ref_noise(*nm_args))
+        assert np.abs(rho.to_numpy() - ref).max() < TOL[prec], (T, seed)
+
+
+def test_c2_density_matrix_config(bt):
+    """Config 2: rho N=8, brickwork 16 layers, depolarizing 1e-3 / 1e-2 and
+    amplitude damping 5e-3 per qubit per layer; complex64 rho vs the
+    ComplexF64 oracle, FP64 trace and purity."""
+    from bluetangle_jl_amd.circuit import Circuit
+    N = 8
+    spec = brickwork_spec(N, 16, 2, ad_gamma=5e-3)
+    nm_args = ("depolarizing", 1e-3, "depolarizing", 1e-2)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
+    rho = bt.zero_rho(N, "c64")
+    stats = c.run_dm(rho)
+    ref = O.run_dm(N, to_ref(spec), O.ref_noise(*nm_args))
+    got = rho.to_numpy()
+    assert np.abs(got - ref).max() < 1e-5
+    tr, pur = rho.trace_purity()
+    assert abs(tr - np.trace(ref).real) < 1e-5
+    assert abs(pur - np.sum(np.abs(ref) ** 2)) < 1e-5
+    assert stats["passes"] > 0
+
+
+def test_c1_to_state(bt):
+    from bluetangle_jl_amd.circuit import Circuit, to_state
+    N = 12
+    spec = brickwork_spec(N, 20, 2026)
+    st = to_state(Circuit(to_bt(spec, bt), N), prec="c128")
+    r, _, _ = oracle_traj(N, spec, None, 0, 0)
+    assert np.abs(st.to_numpy() - r).max() < 1e-12
+    assert np.abs(st.probabilities() - np.abs(r) ** 2).max() < 1e-13
+
+
+def test_run_driver_kats(bt):
+    """run(ops, shots) returns the reference's KATs (runtests.jl:195-231)."""
+    from bluetangle_jl_amd.circuit import run
+    ops_reset = [bt.Op("X", 1), bt.Op("RES", 1), bt.Op("MZ", 1)]
+    ops_ifop = [bt.Op("X", 1), bt.ifOp("MZ", 1, bt.Op("I", 1), bt.Op("X", 1))]
+    ops_mid = [bt.Op("X", 1), bt.Op("X", 2), bt.Op("CX", 2, 3), bt.Op("CX", 1, 2), bt.Op("X", 2), bt.Op("MZ", 2),
+               bt.Op("RES", 2), bt.Op("CX", 2, 3), bt.Op("CX", 1, 2), bt.Op("CX", 2, 3), bt.Op("X", 1)]
+    for batched in (True, False):
+        assert run(ops_reset, 3, batched=batched) == [[0]] * 3
+        assert run(ops_ifop, 3, batched=batched) == [[1]] * 3
+        assert run(ops_mid, 3, batched=batched) == [[1]] * 3

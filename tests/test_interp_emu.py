@@ -1,0 +1,132 @@
+"""CPU checks of the engine's trajectory interpreter (bt_interp.hpp: operator
+queue with fusion, tile planning, lazy/exact branch decisions, folded
+renormalisation) through a host emulation of its passes, against the FP64
+oracle on identical circuits and uniforms.  No GPU involved."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from refcirc import brickwork_spec, random_spec, to_bt, to_ref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EMU_DIR = os.path.join(ROOT, "tests", "emu")
+EMU_SO = os.path.join(EMU_DIR, "libbt_emu.so")
+SRCS = [os.path.join(EMU_DIR, "bt_emu.cpp"), os.path.join(ROOT, "bluetangle.jl_amd", "csrc", "bt_interp.hpp"),
+        os.path.join(ROOT, "bluetangle.jl_amd", "csrc", "bt_circuit.hpp"), os.path.join(ROOT, "include", "bt.h")]
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if not os.path.exists(EMU_SO) or any(os.path.getmtime(s) > os.path.getmtime(EMU_SO) for s in SRCS):
+        subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I" + os.path.join(ROOT, "include"),
+                        "-I" + os.path.join(ROOT, "bluetangle.jl_amd", "csrc"), "-o", EMU_SO, SRCS[0]], check=True)
+    L = ctypes.CDLL(EMU_SO)
+    L.emu_error.restype = ctypes.c_char_p
+    return L
+
+
+def emu_run(emu, bt, spec, N, noise=False, seed=0, traj=0, T=3, cmin=1, state=None, fold=1, tol=1e-9):
+    from bluetangle_jl_amd.circuit import compile_ir
+    ir = compile_ir(to_bt(spec, bt), N, noise)
+    arr, pool = ir.arrays()
+    s = np.zeros(2 ** N, dtype=np.complex128)
+    if state is None:
+        s[0] = 1
+    else:
+        s[:] = state
+    outs = np.zeros(4096, dtype=np.int32)
+    nout = ctypes.c_int64()
+    stats = np.zeros(5, dtype=np.int64)
+    rc = emu.emu_run(N, arr, len(ir.ops), ir.ntop, pool.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(pool.size // 2),
+                     ctypes.c_uint64(seed), ctypes.c_uint64(traj), T, cmin, fold,
+                     s.view(np.float64).ctypes.data_as(ctypes.c_void_p), outs.ctypes.data_as(ctypes.c_void_p),
+                     ctypes.c_int64(4096), ctypes.byref(nout), stats.ctypes.data_as(ctypes.c_void_p),
+                     ctypes.c_double(tol))
+    if rc != 0:
+        raise RuntimeError("emu rc=%d: %s" % (rc, emu.emu_error().decode()))
+    return s, outs[: nout.value].tolist(), stats
+
+
+def oracle_run(spec, N, nm_args=None, seed=0, traj=0, state=None, tol=1e-9):
+    nm = O.ref_noise(*nm_args) if nm_args else None
+    u = O.uniforms(seed, traj, 16 * len(spec) + 16)
+    s, outs, used, bnd, ov = O.run_sv(N, to_ref(spec), nm, u, state=state, tol=tol)
+    return s, outs, used, bnd
+
+
+KINDS = ("gate1", "gate2", "ccx", "chan1", "chan2", "meas", "reset", "ifop")
+
+
+@pytest.mark.parametrize("seed", range(24))
+@pytest.mark.parametrize("T", [2, 3, 5])
+def test_random_noisy_circuits(emu, bt, seed, T):
+    rng = np.random.default_rng(1000 + seed)
+    N = 3 + seed % 4
+    spec = random_spec(N, 60, rng, kinds=KINDS)
+    nm_args = ("depolarizing", 0.05, "amplitude_damping", 0.1)
+    r, routs, used, bnd = oracle_run(spec, N, nm_args, seed)
+    if bnd:
+        pytest.skip("boundary draw")
+    s, outs, stats = emu_run(emu, bt, spec, N, bt.NoiseModel(*nm_args), seed, T=T, cmin=min(1, T))
+    assert outs == routs
+    assert stats[3] == used
+    assert np.array_equal(np.isnan(s), np.isnan(r))
+    assert np.allclose(s, r, rtol=0, atol=1e-11, equal_nan=True)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_brickwork_trajectories(emu, bt, seed):
+    """The trajectory configuration in miniature: brickwork, depolarizing
+    1e-3 / 1e-2, amplitude damping 5e-3 per qubit per layer, MZ with p=0.1."""
+    N = 8
+    spec = brickwork_spec(N, 6, 77, ad_gamma=5e-3, mz_prob=0.3)
+    nm_args = ("depolarizing", 1e-3, "depolarizing", 1e-2)
+    r, routs, used, bnd = oracle_run(spec, N, nm_args, 5, seed)
+    s, outs, stats = emu_run(emu, bt, spec, N, bt.NoiseModel(*nm_args), 5, seed, T=5, cmin=2)
+    assert bnd == 0
+    assert outs == routs
+    assert np.allclose(s, r, rtol=0, atol=1e-12)
+
+
+def test_non_unitary_gates_do_not_renormalise(emu, bt):
+    """Plain gates never renormalise (hilbert.jl:505) even when the matrix is
+    not unitary (P0, SP), while channels and measurements do."""
+    N = 3
+    spec = [("gate", "H", 1, -1, -2, True), ("gate", "H", 2, -1, -2, True), ("chan", "amplitude_damping", 0.3, 1, -1),
+            ("gate", "P0", 2, -1, -2, True), ("gate", "RX(0.3)", 3, -1, -2, True),
+            ("chan", "depolarizing", 0.2, 3, -1), ("gate", "SP", 1, -1, -2, True), ("gate", "H", 3, -1, -2, True)]
+    for seed in range(10):
+        r, routs, used, bnd = oracle_run(spec, N, None, seed)
+        s, outs, stats = emu_run(emu, bt, spec, N, False, seed, T=2, cmin=1)
+        assert np.allclose(s, r, atol=1e-12), seed
+        assert abs(np.linalg.norm(s) - np.linalg.norm(r)) < 1e-12
+
+
+def test_measure_groups_and_resets(emu, bt):
+    N = 5
+    spec = [("gate", "H", q, -1, -2, True) for q in range(1, 6)] + \
+           [("gate", "CNOT", 1, 2, -2, True), ("gate", "CNOT", 3, 4, -2, True)] + \
+           [("meas", "MZ", 1), ("meas", "MZ", 2), ("reset", 3), ("meas", "MZ", 4), ("meas", "MZ", 5),
+            ("meas", "MZ", 3), ("meas", "MX", 2), ("meas", "MR", 4), ("meas", "MY", 1)]
+    for seed in range(20):
+        r, routs, used, bnd = oracle_run(spec, N, None, seed)
+        s, outs, stats = emu_run(emu, bt, spec, N, False, seed, T=3, cmin=1)
+        assert outs == routs, seed
+        assert np.allclose(s, r, atol=1e-12)
+
+
+def test_pass_count_per_layer(emu, bt):
+    """Lazy decisions + fusion keep the trajectory brickwork near two
+    read+write passes per layer."""
+    N, layers = 14, 8
+    spec = brickwork_spec(N, layers, 3, ad_gamma=5e-3, mz_prob=0.1)
+    s, outs, stats = emu_run(emu, bt, spec, N, bt.NoiseModel("depolarizing", 1e-3, "depolarizing", 1e-2), 1, 0,
+                             T=9, cmin=3)
+    passes = stats[0] + 0.5 * stats[1]
+    assert passes <= 3.0 * layers, stats
+    r, routs, used, bnd = oracle_run(spec, N, ("depolarizing", 1e-3, "depolarizing", 1e-2), 1, 0)
+    assert outs == routs and np.allclose(s, r, atol=1e-11)
