@@ -28,7 +28,7 @@ SYMBOLS = [
     "bt_apply_matrix", "bt_reduced_rho", "bt_apply_kraus_traj", "bt_apply_kraus_dm", "bt_measure", "bt_reset",
     "bt_probabilities", "bt_norm2", "bt_trace_purity", "bt_gather",
     "bt_circuit_create", "bt_circuit_destroy", "bt_circuit_max_draws", "bt_run_circuit", "bt_run_circuit_dm",
-    "bt_run_batch", "bt_set_tile_bits",
+    "bt_run_batch", "bt_run_batch_persistent", "bt_set_tile_bits",
     "bt_server_create", "bt_server_submit", "bt_server_poll", "bt_server_fetch", "bt_server_stats",
     "bt_server_destroy",
 ]
@@ -111,6 +111,9 @@ def lib():
             "bt_run_circuit_dm": (ctypes.c_int, [vp, vp, ctypes.POINTER(RunStats)]),
             "bt_run_batch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ip,
                                             ctypes.c_int64, i64p, dp, ctypes.POINTER(RunStats)]),
+            "bt_run_batch_persistent": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int64,
+                                                       ctypes.c_int64, ip, ctypes.c_int64, i64p, dp,
+                                                       ctypes.POINTER(RunStats)]),
             "bt_set_tile_bits": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
             "bt_server_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(vp)]),
             "bt_server_submit": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,

@@ -220,16 +220,18 @@ class Circuit:
         L.check(L.lib().bt_run_circuit_dm(rho.handle, self.handle, ctypes.byref(st)))
         return st.as_dict()
 
-    def run_batch(self, ntraj: int, seed=0, traj0=0, prec="c64", keep_states=False, outcap=None):
-        """ntraj trajectories from |0..0> in one persistent launch.
-        Returns (outcome lists, states or None, stats)."""
+    def run_batch(self, ntraj: int, seed=0, traj0=0, prec="c64", keep_states=False, outcap=None, mode="rounds"):
+        """ntraj trajectories from |0..0>: mode "rounds" (lockstep launches over
+        all resident trajectories) or "persistent" (one launch, device-side
+        interpreters).  Returns (outcome lists, states or None, stats)."""
         from .core import PRECISIONS
         outcap = outcap or max(16, 2 * self.nops)
         outs = np.zeros((ntraj, outcap), dtype=np.int32)
         nouts = np.zeros(ntraj, dtype=np.int64)
         states = np.zeros((ntraj, 2 ** self.N), dtype=np.complex128) if keep_states else None
         st = L.RunStats()
-        L.check(L.lib().bt_run_batch(self.handle, PRECISIONS[prec], seed, traj0, ntraj, L.iptr(outs), outcap,
+        fn = L.lib().bt_run_batch if mode == "rounds" else L.lib().bt_run_batch_persistent
+        L.check(fn(self.handle, PRECISIONS[prec], seed, traj0, ntraj, L.iptr(outs), outcap,
                                      nouts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                      L.dptr(states.view(np.float64)) if keep_states else None,
                                      ctypes.byref(st)))

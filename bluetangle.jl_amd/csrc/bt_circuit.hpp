@@ -4,8 +4,11 @@
 // (unitary or not).
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -244,4 +247,124 @@ inline int analyze(int nbits, const bt_ir_op* ops, int nops, int ntop, const dou
   return BT_OK;
 }
 
+using cd = std::complex<double>;
+
+// Density-matrix lowering: every op becomes superoperators on vec(rho)
+// (row bit b at b, column bit b at b + N).  A gate followed by its noise
+// channel on the same bits becomes one superoperator sum_i (K_i U) (x)
+// conj(K_i U); other gates become U on the row bits and conj(U) on the column
+// bits (controls on both sides).
+inline int lower_dm(const CircuitHost& c, CircuitHost* out, std::string* msg) {
+  const int N = c.n;
+  out->n = 2 * N;
+  out->ops.clear();
+  out->pool.clear();
+  out->chanw.assign(1, 0.0);
+  auto add_gate = [&](int k, const int32_t* bits, uint64_t ctrl, const std::vector<cd>& m) {
+    bt_ir_op o;
+    memset(&o, 0, sizeof o);
+    o.kind = BT_OP_GATE;
+    o.k = k;
+    for (int i = 0; i < k; i++) o.bits[i] = bits[i];
+    o.ctrl_mask = ctrl;
+    o.mat = (int64_t)(out->pool.size() / 2);
+    for (const cd& v : m) {
+      out->pool.push_back(v.real());
+      out->pool.push_back(v.imag());
+    }
+    out->ops.push_back(o);
+  };
+  auto superop = [&](int k, const std::vector<std::vector<cd>>& A, const int32_t* bits) {
+    const int D = 1 << k, S = D * D;
+    std::vector<cd> sup(S * S, cd(0, 0));
+    for (const auto& K : A)
+      for (int r1 = 0; r1 < D; r1++)
+        for (int c1 = 0; c1 < D; c1++)
+          for (int r0 = 0; r0 < D; r0++)
+            for (int c0 = 0; c0 < D; c0++) {
+              cd v = K[r1 + r0 * D] * std::conj(K[c1 + c0 * D]);
+              sup[((r1 << k) | c1) + ((r0 << k) | c0) * S] += v;
+            }
+    int32_t sb[4];
+    for (int i = 0; i < k; i++) {
+      sb[i] = bits[i];
+      sb[k + i] = bits[i] + N;
+    }
+    add_gate(2 * k, sb, 0, sup);
+  };
+  auto mat = [&](const bt_ir_op& op, int idx) {
+    const int D = 1 << op.k;
+    std::vector<cd> m(D * D);
+    const double* p = c.pool.data() + 2 * (op.mat + (int64_t)idx * D * D);
+    for (int i = 0; i < D * D; i++) m[i] = cd(p[2 * i], p[2 * i + 1]);
+    return m;
+  };
+  for (int i = 0; i < c.ntop; i++) {
+    const bt_ir_op& op = c.ops[i];
+    if (op.kind == BT_OP_MEASURE || op.kind == BT_OP_IF) {
+      *msg = "fix this: density-matrix measurement is not implemented";
+      return BT_ERR_UNSUPPORTED;
+    }
+    if (op.kind == BT_OP_RESET) {  // OpQC("RES") on rho: amplitude damping with gamma = 1
+      std::vector<cd> k0 = {1, 0, 0, 0}, k1 = {0, 0, 1, 0};
+      superop(1, {k0, k1}, op.bits);
+      continue;
+    }
+    if (op.kind == BT_OP_CHANNEL) {
+      if (op.k > 2) {
+        *msg = "3-qubit channels have no density-matrix method";
+        return BT_ERR_UNSUPPORTED;
+      }
+      std::vector<std::vector<cd>> A;
+      for (int j = 0; j < op.nkraus; j++) A.push_back(mat(op, j));
+      superop(op.k, A, op.bits);
+      continue;
+    }
+    // gate
+    if (op.k > 2) {
+      *msg = "density-matrix gates take k <= 2";
+      return BT_ERR_UNSUPPORTED;
+    }
+    const std::vector<cd> U = mat(op, 0);
+    const int D = 1 << op.k;
+    if (op.ctrl_mask == 0 && i + 1 < c.ntop && c.ops[i + 1].kind == BT_OP_CHANNEL && c.ops[i + 1].k == op.k &&
+        std::equal(op.bits, op.bits + op.k, c.ops[i + 1].bits)) {
+      const bt_ir_op& ch = c.ops[i + 1];
+      std::vector<std::vector<cd>> A;
+      for (int j = 0; j < ch.nkraus; j++) {
+        std::vector<cd> K = mat(ch, j), KU(D * D, cd(0, 0));
+        for (int r = 0; r < D; r++)
+          for (int cc = 0; cc < D; cc++)
+            for (int x = 0; x < D; x++) KU[r + cc * D] += K[r + x * D] * U[x + cc * D];
+        A.push_back(KU);
+      }
+      superop(op.k, A, op.bits);
+      i++;
+      continue;
+    }
+    std::vector<cd> Uc(U.size());
+    for (size_t j = 0; j < U.size(); j++) Uc[j] = std::conj(U[j]);
+    int32_t cb[4];
+    for (int j = 0; j < op.k; j++) cb[j] = op.bits[j] + N;
+    add_gate(op.k, op.bits, op.ctrl_mask, U);
+    add_gate(op.k, cb, op.ctrl_mask << N, Uc);
+  }
+  out->ntop = (int)out->ops.size();
+  out->chan.assign(out->ops.size(), bti::ChanDesc{1, 0, 0});  // no renormalisation bookkeeping
+  out->max_draws = 0;
+  out->has_measure = false;
+  return BT_OK;
+}
+
 }  // namespace btc
+
+// A validated circuit and its device copies (uploaded on first use).
+struct bt_circuit_s {
+  btc::CircuitHost h;
+  // device copies (uploaded on first use)
+  bt_ir_op* d_ops = nullptr;
+  double* d_pool = nullptr;
+  bti::ChanDesc* d_chan = nullptr;
+  double* d_chanw = nullptr;
+  std::mutex mu;
+};

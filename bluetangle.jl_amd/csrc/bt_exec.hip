@@ -23,331 +23,9 @@
 #include "bt_circuit.hpp"
 #include "bt_internal.hpp"
 #include "bt_interp.hpp"
-
-using bti::Plan;
-using bti::PassResult;
-using bti::QOp;
+#include "bt_tile.hpp"
 
 namespace bt {
-
-constexpr int kMatEntries = bti::Interp::kPassMatEntries;
-constexpr int kAcc = 33;  // norm2, hist[16], packed rho[16]
-
-struct LOp {
-  int32_t k;
-  int32_t lsorted[4];  // ascending local target positions
-  uint32_t loff[16];   // tile offset of each local matrix index
-  uint32_t lctrl;      // control bits inside the tile
-  uint64_t gctrl;      // control bits outside the tile (global positions)
-  int32_t moff;        // matrix offset in the pass's matrix table
-};
-
-struct PassGeom {
-  int32_t T, c0, nA, nB, ncomp, nops, red, red_k, write, need_norm;
-  int32_t tile[bti::kMaxT];
-  int32_t comp[40];
-  int32_t red_bits[4];  // RED_HIST: global bits; RED_RHO: local positions, MSB first
-  double scale;
-  LOp ops[bti::kPMax];
-};
-
-// Host/device: turn a plan + queue slots into the tile-local operator table.
-template <typename R>
-__host__ __device__ __attribute__((noinline)) void build_geom(const Plan& plan, const QOp* slots, int nbits, PassGeom* g, C<R>* mats) {
-  g->T = plan.ntile;
-  for (int i = 0; i < plan.ntile; i++) g->tile[i] = plan.tile[i];
-  int c0 = 0;
-  while (c0 < plan.ntile && plan.tile[c0] == c0) c0++;
-  g->c0 = c0;
-  const int nhigh = plan.ntile - c0;
-  g->nA = nhigh < 7 ? nhigh : 7;
-  g->nB = nhigh - g->nA;
-  g->ncomp = 0;
-  uint64_t tmask = 0;
-  for (int i = 0; i < plan.ntile; i++) tmask |= 1ull << plan.tile[i];
-  for (int b = 0; b < nbits; b++)
-    if (!((tmask >> b) & 1)) g->comp[g->ncomp++] = b;
-  int rank[64];
-  for (int b = 0; b < 64; b++) rank[b] = -1;
-  for (int i = 0; i < plan.ntile; i++) rank[plan.tile[i]] = i;
-  g->nops = plan.nops;
-  g->write = plan.write;
-  g->need_norm = (plan.aligned || plan.red != bti::RED_NONE) ? 1 : 0;
-  g->scale = plan.scale;
-  int moff = 0;
-  for (int o = 0; o < plan.nops; o++) {
-    const QOp& q = slots[plan.qidx[o]];
-    LOp& L = g->ops[o];
-    L.k = q.k;
-    int lb[4];
-    for (int i = 0; i < q.k; i++) lb[i] = rank[q.bits[i]];
-    for (int i = 0; i < q.k; i++) L.lsorted[i] = lb[i];
-    for (int i = 0; i < q.k; i++)
-      for (int j = i + 1; j < q.k; j++)
-        if (L.lsorted[j] < L.lsorted[i]) {
-          int t = L.lsorted[i];
-          L.lsorted[i] = L.lsorted[j];
-          L.lsorted[j] = t;
-        }
-    const int D = 1 << q.k;
-    for (int l = 0; l < D; l++) {
-      uint32_t o2 = 0;
-      for (int i = 0; i < q.k; i++)
-        if ((l >> (q.k - 1 - i)) & 1) o2 |= 1u << lb[i];
-      L.loff[l] = o2;
-    }
-    L.lctrl = 0;
-    L.gctrl = 0;
-    for (int b = 0; b < nbits; b++)
-      if ((q.ctrl >> b) & 1) {
-        if (rank[b] >= 0) L.lctrl |= 1u << rank[b];
-        else L.gctrl |= 1ull << b;
-      }
-    L.moff = moff;
-    for (int i = 0; i < D * D; i++) {
-      mats[moff + i].x = (R)q.m[2 * i];
-      mats[moff + i].y = (R)q.m[2 * i + 1];
-    }
-    moff += D * D;
-  }
-  g->red = plan.red;
-  g->red_k = plan.red_k;
-  for (int i = 0; i < plan.red_k; i++)
-    g->red_bits[i] = plan.red == bti::RED_RHO ? rank[plan.red_bits[i]] : plan.red_bits[i];
-}
-
-__device__ __forceinline__ uint32_t ins0(uint32_t x, int b) { return ((x >> b) << (b + 1)) | (x & ((1u << b) - 1u)); }
-
-template <typename R, int K, int NT>
-__device__ __forceinline__ void apply_lop(C<R>* tile, int T, const LOp& op, const C<R>* mats) {
-  constexpr int D = 1 << K;
-  const uint32_t ng = 1u << (T - K);
-  const C<R>* m = mats + op.moff;
-  for (uint32_t g = threadIdx.x; g < ng; g += NT) {
-    uint32_t lb = g;
-#pragma unroll
-    for (int i = 0; i < K; i++) lb = ins0(lb, op.lsorted[i]);
-    if ((lb & op.lctrl) != op.lctrl) continue;
-    C<R> v[D];
-#pragma unroll
-    for (int l = 0; l < D; l++) v[l] = tile[lb + op.loff[l]];
-    if (K <= 2) {
-#pragma unroll
-      for (int r = 0; r < D; r++) {
-        C<R> acc = cmake<C<R>>(0, 0);
-#pragma unroll
-        for (int c = 0; c < D; c++) acc = cfma(acc, m[r + c * D], v[c]);
-        tile[lb + op.loff[r]] = acc;
-      }
-    } else {
-      // volatile reads keep the D*D matrix entries (LDS broadcasts) inside
-      // the group loop instead of hoisting them all into registers; every v
-      // is read before any write
-      const volatile R* mv = (const volatile R*)m;
-      C<R> w[D];
-#pragma unroll
-      for (int r = 0; r < D; r++) {
-        C<R> acc = cmake<C<R>>(0, 0);
-#pragma unroll
-        for (int c = 0; c < D; c++) acc = cfma(acc, cmake<C<R>>(mv[2 * (r + c * D)], mv[2 * (r + c * D) + 1]), v[c]);
-        w[r] = acc;
-      }
-#pragma unroll
-      for (int r = 0; r < D; r++) tile[lb + op.loff[r]] = w[r];
-    }
-  }
-}
-
-// Block-wide sum of M per-thread values, added to sums[0..M) (LDS) in a
-// fixed order (waves, then lanes by xor butterfly): bit-reproducible.
-template <int M, int NT>
-__device__ __forceinline__ void block_add(const double* v, double* red /*LDS [NT/64][M]*/, double* sums) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll 1
-  for (int j = 0; j < M; j++) {
-    double x = v[j];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if (lane == 0) red[wave * M + j] = x;
-  }
-  __syncthreads();
-  if (threadIdx.x < M) {
-    double t = 0;
-    for (int w = 0; w < NT / 64; w++) t += red[w * M + threadIdx.x];
-    sums[threadIdx.x] += t;
-  }
-  __syncthreads();
-}
-
-// Packed reduced rho over RK local bits (MSB first) of the tile, added to
-// racc: diagonal, then Re/Im of rho[a,b] for a < b.
-template <typename R, int RK, int NT>
-__device__ __forceinline__ void rho_accumulate(const C<R>* tile, int T, const int32_t* rbits, double* racc) {
-  constexpr int D = 1 << RK;
-  int srt[RK];
-#pragma unroll
-  for (int i = 0; i < RK; i++) srt[i] = rbits[i];
-  if (RK > 1 && srt[RK - 1] < srt[0]) {
-    int x = srt[0];
-    srt[0] = srt[RK - 1];
-    srt[RK - 1] = x;
-  }
-  uint32_t roff[D];
-#pragma unroll
-  for (int a = 0; a < D; a++) {
-    uint32_t o2 = 0;
-#pragma unroll
-    for (int i = 0; i < RK; i++)
-      if ((a >> (RK - 1 - i)) & 1) o2 |= 1u << rbits[i];
-    roff[a] = o2;
-  }
-  const uint32_t ng = 1u << (T - RK);
-  for (uint32_t gi = threadIdx.x; gi < ng; gi += NT) {
-    uint32_t lb = gi;
-#pragma unroll
-    for (int i = 0; i < RK; i++) lb = ins0(lb, srt[i]);
-    double vr[D], vi[D];
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-      C<R> v = tile[lb + roff[a]];
-      vr[a] = (double)v.x;
-      vi[a] = (double)v.y;
-    }
-    int p = D;
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-      racc[a] += vr[a] * vr[a] + vi[a] * vi[a];
-#pragma unroll
-      for (int b = a + 1; b < D; b++) {
-        racc[p] += vr[a] * vr[b] + vi[a] * vi[b];
-        racc[p + 1] += vi[a] * vr[b] - vr[a] * vi[b];
-        p += 2;
-      }
-    }
-  }
-}
-
-// One pass over tiles t = t0, t0 + tstep, ... < ntiles of `state`.
-// Reductions accumulate per tile and are added to sums[0..kAcc) (LDS, zeroed
-// by the caller): [0] norm^2, [1..16] histogram, [17..32] packed rho.
-template <typename R, int NT>
-__device__ void tile_pass(C<R>* __restrict__ state, const PassGeom& g, const C<R>* mats, C<R>* tile,
-                          const uint64_t* hA, const uint64_t* hB, uint64_t t0, uint64_t tstep, uint64_t ntiles,
-                          double* red, double* sums) {
-  const int T = g.T;
-  const uint32_t TL = 1u << T;
-  const uint32_t lowmask = (1u << g.c0) - 1u;
-  const uint32_t maskA = (1u << g.nA) - 1u;
-  const int shB = g.c0 + g.nA;
-  const R scale = (R)g.scale;
-  for (uint64_t t = t0; t < ntiles; t += tstep) {
-    uint64_t base = 0, tt = t;
-    for (int j = 0; j < g.ncomp; j++) {
-      base |= (tt & 1ull) << g.comp[j];
-      tt >>= 1;
-    }
-    for (uint32_t l = threadIdx.x; l < TL; l += NT) {
-      const uint64_t off = (uint64_t)(l & lowmask) | hA[(l >> g.c0) & maskA] | hB[l >> shB];
-      C<R> v = state[base + off];
-      v.x *= scale;
-      v.y *= scale;
-      tile[l] = v;
-    }
-    __syncthreads();
-    for (int o = 0; o < g.nops; o++) {
-      const LOp& op = g.ops[o];
-      if ((base & op.gctrl) != op.gctrl) continue;  // uniform over the block
-      switch (op.k) {
-        case 1: apply_lop<R, 1, NT>(tile, T, op, mats); break;
-        case 2: apply_lop<R, 2, NT>(tile, T, op, mats); break;
-        case 3: apply_lop<R, 3, NT>(tile, T, op, mats); break;
-        default: apply_lop<R, 4, NT>(tile, T, op, mats); break;
-      }
-      __syncthreads();
-    }
-    if (g.red == bti::RED_RHO) {
-      double racc[16];
-#pragma unroll
-      for (int j = 0; j < 16; j++) racc[j] = 0.0;
-      if (g.red_k == 1) rho_accumulate<R, 1, NT>(tile, T, g.red_bits, racc);
-      else rho_accumulate<R, 2, NT>(tile, T, g.red_bits, racc);
-      block_add<16, NT>(racc, red, sums + 17);
-    }
-    if (g.red == bti::RED_HIST) {
-      double hacc[17];
-#pragma unroll
-      for (int j = 0; j < 17; j++) hacc[j] = 0.0;
-      for (uint32_t l = threadIdx.x; l < TL; l += NT) {
-        const uint64_t off = (uint64_t)(l & lowmask) | hA[(l >> g.c0) & maskA] | hB[l >> shB];
-        const C<R> v = tile[l];
-        if (g.write) state[base + off] = v;
-        const double w = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-        hacc[0] += w;
-        const uint64_t gi = base | off;
-        int bin = 0;
-        for (int j = 0; j < g.red_k; j++) bin |= (int)((gi >> g.red_bits[j]) & 1ull) << j;
-#pragma unroll
-        for (int b = 0; b < 16; b++) hacc[1 + b] += (bin == b) ? w : 0.0;
-      }
-      block_add<17, NT>(hacc, red, sums);
-    } else if (g.need_norm) {
-      double nacc[1] = {0.0};
-      for (uint32_t l = threadIdx.x; l < TL; l += NT) {
-        const uint64_t off = (uint64_t)(l & lowmask) | hA[(l >> g.c0) & maskA] | hB[l >> shB];
-        const C<R> v = tile[l];
-        if (g.write) state[base + off] = v;
-        nacc[0] += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-      }
-      block_add<1, NT>(nacc, red, sums);
-    } else {
-      for (uint32_t l = threadIdx.x; l < TL; l += NT) {
-        const uint64_t off = (uint64_t)(l & lowmask) | hA[(l >> g.c0) & maskA] | hB[l >> shB];
-        if (g.write) state[base + off] = tile[l];
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Fill the offset tables of the tile's high bits (all threads).
-template <int NT>
-__device__ __forceinline__ void build_tables(const PassGeom& g, uint64_t* hA, uint64_t* hB) {
-  for (int h = threadIdx.x; h < (1 << g.nA); h += NT) {
-    uint64_t o = 0;
-    for (int j = 0; j < g.nA; j++)
-      if ((h >> j) & 1) o |= 1ull << g.tile[g.c0 + j];
-    hA[h] = o;
-  }
-  for (int h = threadIdx.x; h < (1 << g.nB); h += NT) {
-    uint64_t o = 0;
-    for (int j = 0; j < g.nB; j++)
-      if ((h >> j) & 1) o |= 1ull << g.tile[g.c0 + g.nA + j];
-    hB[h] = o;
-  }
-}
-
-// Packed (diag, then a<b re/im) rho over k bits -> column-major complex.
-__host__ __device__ inline void unpack_rho(int k, const double* packed, double* rho) {
-  const int D = 1 << k;
-  for (int i = 0; i < 2 * D * D; i++) rho[i] = 0.0;
-  for (int a = 0; a < D; a++) rho[2 * (a + a * D)] = packed[a];
-  int p = D;
-  for (int a = 0; a < D; a++)
-    for (int b = a + 1; b < D; b++) {
-      rho[2 * (a + b * D)] = packed[p];
-      rho[2 * (a + b * D) + 1] = packed[p + 1];
-      rho[2 * (b + a * D)] = packed[p];
-      rho[2 * (b + a * D) + 1] = -packed[p + 1];
-      p += 2;
-    }
-}
-
-__host__ __device__ inline void to_result(const Plan& plan, const double* sum, PassResult* res) {
-  res->norm2 = sum[0];
-  for (int j = 0; j < 16; j++) res->hist[j] = sum[1 + j];
-  for (int j = 0; j < 32; j++) res->rho[j] = 0.0;
-  if (plan.red == bti::RED_RHO) unpack_rho(plan.red_k, sum + 17, res->rho);
-}
 
 // ---------------------------------------------------------------------------
 // persistent batched trajectories
@@ -371,13 +49,13 @@ struct BatchArgs {
   int64_t max_passes;
 };
 
-template <typename R, int NT>
-__global__ void __launch_bounds__(NT) k_batch(BatchArgs a) {
+template <typename R>
+__global__ void __launch_bounds__(TileCfg<R>::NT) k_batch(BatchArgs a) {
+  constexpr int NT = TileCfg<R>::NT;
   extern __shared__ __align__(16) unsigned char smem[];
   C<R>* tile = (C<R>*)smem;
   __shared__ PassGeom geom;
   __shared__ C<R> mats[kMatEntries];
-  __shared__ uint64_t hA[128], hB[256];
   __shared__ double red[(NT / 64) * 17];
   __shared__ double sums[kAcc];
   __shared__ Plan plan;
@@ -395,7 +73,6 @@ __global__ void __launch_bounds__(NT) k_batch(BatchArgs a) {
     __syncthreads();
     const unsigned long long t = traj_sh;
     if (t >= (unsigned long long)a.ntraj) break;
-    for (uint64_t i = threadIdx.x; i < len; i += NT) state[i] = cmake<C<R>>(i == 0 ? (R)1 : (R)0, (R)0);
     if (threadIdx.x == 0) {
       ip->init(a.prog, a.slots + (uint64_t)blockIdx.x * bti::kQMax, a.T, a.cmin);
       ip->seed = a.seed;
@@ -407,6 +84,7 @@ __global__ void __launch_bounds__(NT) k_batch(BatchArgs a) {
     }
     __syncthreads();
     int64_t npass = 0;
+    int fresh = 1;
     for (;;) {
       if (threadIdx.x == 0) {
         int c = ip->step(&plan, &res);
@@ -414,16 +92,22 @@ __global__ void __launch_bounds__(NT) k_batch(BatchArgs a) {
           ip->err = BT_ERR_STATE;
           c = 0;
         }
-        if (c) build_geom<R>(plan, ip->slots, a.nbits, &geom, mats);
+        if (c) {
+          const int rc = build_geom<R>(plan, ip->slots, a.nbits, &geom, mats);
+          if (rc) {
+            ip->err = rc;
+            c = 0;
+          }
+        }
         cmd = c;
       }
       __syncthreads();
       if (!cmd) break;
-      build_tables<NT>(geom, hA, hB);
       if (threadIdx.x < kAcc) sums[threadIdx.x] = 0.0;
       __syncthreads();
-      tile_pass<R, NT>(state, geom, mats, tile, hA, hB, 0, 1, ntiles, red, sums);
-      if (threadIdx.x == 0) to_result(plan, sums, &res);
+      tile_pass<R>(state, geom, mats, tile, 0, 1, ntiles, red, sums, fresh);
+      if (geom.write) fresh = 0;
+      if (threadIdx.x == 0) to_result(plan, geom.red_swap, sums, &res);
       __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -438,7 +122,7 @@ __global__ void __launch_bounds__(NT) k_batch(BatchArgs a) {
     if (a.want_final) {
       double* out = a.final_states + (uint64_t)t * 2 * len;
       for (uint64_t i = threadIdx.x; i < len; i += NT) {
-        C<R> v = state[i];
+        C<R> v = fresh ? cmake<C<R>>(i == 0 ? (R)1 : (R)0, (R)0) : state[i];
         out[2 * i] = (double)v.x;
         out[2 * i + 1] = (double)v.y;
       }
@@ -456,27 +140,26 @@ struct GridTable {
   C<R> mats[kMatEntries];
 };
 
-template <typename R, int NT>
-__global__ void __launch_bounds__(NT) k_grid_pass(C<R>* state, const GridTable<R>* tab, uint64_t ntiles,
-                                                  double* part) {
+template <typename R>
+__global__ void __launch_bounds__(TileCfg<R>::NT) k_grid_pass(C<R>* state, const GridTable<R>* tab, uint64_t ntiles,
+                                                              double* part) {
+  constexpr int NT = TileCfg<R>::NT;
   extern __shared__ __align__(16) unsigned char smem[];
   C<R>* tile = (C<R>*)smem;
   __shared__ PassGeom geom;
   __shared__ C<R> mats[kMatEntries];
-  __shared__ uint64_t hA[128], hB[256];
   __shared__ double red[(NT / 64) * 17];
   __shared__ double sums[kAcc];
   {
     const int* src = (const int*)&tab->g;
     int* dst = (int*)&geom;
     for (int i = threadIdx.x; i < (int)(sizeof(PassGeom) / sizeof(int)); i += NT) dst[i] = src[i];
-    for (int i = threadIdx.x; i < kMatEntries; i += NT) mats[i] = tab->mats[i];
+    const int nm = tab->g.nmats;
+    for (int i = threadIdx.x; i < nm; i += NT) mats[i] = tab->mats[i];
   }
-  __syncthreads();
-  build_tables<NT>(geom, hA, hB);
   if (threadIdx.x < kAcc) sums[threadIdx.x] = 0.0;
   __syncthreads();
-  tile_pass<R, NT>(state, geom, mats, tile, hA, hB, blockIdx.x, gridDim.x, ntiles, red, sums);
+  tile_pass<R>(state, geom, mats, tile, blockIdx.x, gridDim.x, ntiles, red, sums);
   if (threadIdx.x < kAcc) part[(uint64_t)blockIdx.x * kAcc + threadIdx.x] = sums[threadIdx.x];
 }
 
@@ -493,15 +176,6 @@ __global__ void k_finalize_acc(const double* part, int nblocks, double* out) {
 // ---------------------------------------------------------------------------
 // circuits
 // ---------------------------------------------------------------------------
-struct bt_circuit_s {
-  btc::CircuitHost h;
-  // device copies (uploaded on first use)
-  bt_ir_op* d_ops = nullptr;
-  double* d_pool = nullptr;
-  bti::ChanDesc* d_chan = nullptr;
-  double* d_chanw = nullptr;
-  std::mutex mu;
-};
 
 namespace bt {
 
@@ -527,14 +201,15 @@ static int upload_circuit(bt_circuit_s* c) {
 // 512-thread workgroup, or an override from the autotuner.
 static int g_tile_override[2] = {0, 0};
 
-int default_tile_bits(int prec) { return prec == BT_C64 ? 14 : 13; }
+int default_tile_bits(int prec) {
+  return prec == BT_C64 ? TileCfg<float>::LOGNT + TileCfg<float>::RB : TileCfg<double>::LOGNT + TileCfg<double>::RB;
+}
 
 int tile_bits_for(int prec, int nbits) {
   int T = g_tile_override[prec] ? g_tile_override[prec] : default_tile_bits(prec);
   return std::min(T, nbits);
 }
 
-constexpr int kNT = 512;
 
 // Host driver of the interpreter over grid passes on state s (SV or DM).
 static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, uint64_t traj, const double* uni,
@@ -559,7 +234,7 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
   const size_t tabsz = s->prec == BT_C64 ? sizeof(GridTable<float>) : sizeof(GridTable<double>);
   const uint64_t ntiles = 1ull << (nbits - std::min(T, nbits));
   const int cus = ctx().cus > 0 ? ctx().cus : 256;
-  const unsigned blocks = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)cus * 2);
+  const unsigned blocks = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)cus * 8);
   const size_t lds = (size_t)(1u << std::min(T, nbits)) * s->elem();
   constexpr int kRing = 8;
   void* h_ring = nullptr;
@@ -582,10 +257,10 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
     rc = fail(BT_ERR_HIP, "event");
   if (rc == BT_OK) {
     if (s->prec == BT_C64)
-      (void)hipFuncSetAttribute((const void*)k_grid_pass<float, kNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (void)hipFuncSetAttribute((const void*)k_grid_pass<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
     else
-      (void)hipFuncSetAttribute((const void*)k_grid_pass<double, kNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (void)hipFuncSetAttribute((const void*)k_grid_pass<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
     (void)hipEventRecord(e0, s->stream);
   }
@@ -593,6 +268,7 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
   PassResult res;
   memset(&res, 0, sizeof res);
   int64_t npass = 0;
+  int red_swap = 0;
   const int64_t max_passes = 64 + 8 * (int64_t)ch.ops.size();
   while (rc == BT_OK && ip->step(&plan, &res)) {
     if (++npass > max_passes) {
@@ -606,12 +282,19 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
     }
     char* hslot = (char*)h_ring + tabsz * slot;
     char* dslot = (char*)d_ring + tabsz * slot;
+    int grc;
     if (s->prec == BT_C64) {
       auto* t = (GridTable<float>*)hslot;
-      build_geom<float>(plan, slots.data(), nbits, &t->g, t->mats);
+      grc = build_geom<float>(plan, slots.data(), nbits, &t->g, t->mats);
+      red_swap = t->g.red_swap;
     } else {
       auto* t = (GridTable<double>*)hslot;
-      build_geom<double>(plan, slots.data(), nbits, &t->g, t->mats);
+      grc = build_geom<double>(plan, slots.data(), nbits, &t->g, t->mats);
+      red_swap = t->g.red_swap;
+    }
+    if (grc) {
+      rc = fail(grc, "run_grid: pass geometry");
+      break;
     }
     if (hipMemcpyAsync(dslot, hslot, tabsz, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
       rc = fail(BT_ERR_HIP, "run_grid: table copy");
@@ -619,11 +302,11 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
     }
     (void)hipEventRecord(ev[slot], s->stream);
     if (s->prec == BT_C64)
-      hipLaunchKernelGGL((k_grid_pass<float, kNT>), dim3(blocks), dim3(kNT), lds, s->stream, (C<float>*)s->d,
-                         (const GridTable<float>*)dslot, ntiles, d_part);
+      hipLaunchKernelGGL((k_grid_pass<float>), dim3(blocks), dim3(TileCfg<float>::NT), lds, s->stream,
+                         (C<float>*)s->d, (const GridTable<float>*)dslot, ntiles, d_part);
     else
-      hipLaunchKernelGGL((k_grid_pass<double, kNT>), dim3(blocks), dim3(kNT), lds, s->stream, (C<double>*)s->d,
-                         (const GridTable<double>*)dslot, ntiles, d_part);
+      hipLaunchKernelGGL((k_grid_pass<double>), dim3(blocks), dim3(TileCfg<double>::NT), lds, s->stream,
+                         (C<double>*)s->d, (const GridTable<double>*)dslot, ntiles, d_part);
     if (hipGetLastError() != hipSuccess) {
       rc = fail(BT_ERR_HIP, "run_grid: pass launch");
       break;
@@ -635,7 +318,7 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
         rc = fail(BT_ERR_HIP, "run_grid: reduction copy");
         break;
       }
-      to_result(plan, h_acc, &res);
+      to_result(plan, red_swap, h_acc, &res);
     } else {
       res.norm2 = 0.0;  // not used for an intermediate pass
     }
@@ -666,115 +349,6 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
   if (d_part) (void)hipFree(d_part);
   if (d_acc) (void)hipFree(d_acc);
   return rc;
-}
-
-using cd = std::complex<double>;
-
-// Density-matrix lowering: every op becomes superoperators on vec(rho)
-// (row bit b at b, column bit b at b + N).  A gate followed by its noise
-// channel on the same bits becomes one superoperator sum_i (K_i U) (x)
-// conj(K_i U); other gates become U on the row bits and conj(U) on the column
-// bits (controls on both sides).
-static int lower_dm(const btc::CircuitHost& c, btc::CircuitHost* out, std::string* msg) {
-  const int N = c.n;
-  out->n = 2 * N;
-  out->ops.clear();
-  out->pool.clear();
-  out->chanw.assign(1, 0.0);
-  auto add_gate = [&](int k, const int32_t* bits, uint64_t ctrl, const std::vector<cd>& m) {
-    bt_ir_op o;
-    memset(&o, 0, sizeof o);
-    o.kind = BT_OP_GATE;
-    o.k = k;
-    for (int i = 0; i < k; i++) o.bits[i] = bits[i];
-    o.ctrl_mask = ctrl;
-    o.mat = (int64_t)(out->pool.size() / 2);
-    for (const cd& v : m) {
-      out->pool.push_back(v.real());
-      out->pool.push_back(v.imag());
-    }
-    out->ops.push_back(o);
-  };
-  auto superop = [&](int k, const std::vector<std::vector<cd>>& A, const int32_t* bits) {
-    const int D = 1 << k, S = D * D;
-    std::vector<cd> sup(S * S, cd(0, 0));
-    for (const auto& K : A)
-      for (int r1 = 0; r1 < D; r1++)
-        for (int c1 = 0; c1 < D; c1++)
-          for (int r0 = 0; r0 < D; r0++)
-            for (int c0 = 0; c0 < D; c0++) {
-              cd v = K[r1 + r0 * D] * std::conj(K[c1 + c0 * D]);
-              sup[((r1 << k) | c1) + ((r0 << k) | c0) * S] += v;
-            }
-    int32_t sb[4];
-    for (int i = 0; i < k; i++) {
-      sb[i] = bits[i];
-      sb[k + i] = bits[i] + N;
-    }
-    add_gate(2 * k, sb, 0, sup);
-  };
-  auto mat = [&](const bt_ir_op& op, int idx) {
-    const int D = 1 << op.k;
-    std::vector<cd> m(D * D);
-    const double* p = c.pool.data() + 2 * (op.mat + (int64_t)idx * D * D);
-    for (int i = 0; i < D * D; i++) m[i] = cd(p[2 * i], p[2 * i + 1]);
-    return m;
-  };
-  for (int i = 0; i < c.ntop; i++) {
-    const bt_ir_op& op = c.ops[i];
-    if (op.kind == BT_OP_MEASURE || op.kind == BT_OP_IF) {
-      *msg = "fix this: density-matrix measurement is not implemented";
-      return BT_ERR_UNSUPPORTED;
-    }
-    if (op.kind == BT_OP_RESET) {  // OpQC("RES") on rho: amplitude damping with gamma = 1
-      std::vector<cd> k0 = {1, 0, 0, 0}, k1 = {0, 0, 1, 0};
-      superop(1, {k0, k1}, op.bits);
-      continue;
-    }
-    if (op.kind == BT_OP_CHANNEL) {
-      if (op.k > 2) {
-        *msg = "3-qubit channels have no density-matrix method";
-        return BT_ERR_UNSUPPORTED;
-      }
-      std::vector<std::vector<cd>> A;
-      for (int j = 0; j < op.nkraus; j++) A.push_back(mat(op, j));
-      superop(op.k, A, op.bits);
-      continue;
-    }
-    // gate
-    if (op.k > 2) {
-      *msg = "density-matrix gates take k <= 2";
-      return BT_ERR_UNSUPPORTED;
-    }
-    const std::vector<cd> U = mat(op, 0);
-    const int D = 1 << op.k;
-    if (op.ctrl_mask == 0 && i + 1 < c.ntop && c.ops[i + 1].kind == BT_OP_CHANNEL && c.ops[i + 1].k == op.k &&
-        std::equal(op.bits, op.bits + op.k, c.ops[i + 1].bits)) {
-      const bt_ir_op& ch = c.ops[i + 1];
-      std::vector<std::vector<cd>> A;
-      for (int j = 0; j < ch.nkraus; j++) {
-        std::vector<cd> K = mat(ch, j), KU(D * D, cd(0, 0));
-        for (int r = 0; r < D; r++)
-          for (int cc = 0; cc < D; cc++)
-            for (int x = 0; x < D; x++) KU[r + cc * D] += K[r + x * D] * U[x + cc * D];
-        A.push_back(KU);
-      }
-      superop(op.k, A, op.bits);
-      i++;
-      continue;
-    }
-    std::vector<cd> Uc(U.size());
-    for (size_t j = 0; j < U.size(); j++) Uc[j] = std::conj(U[j]);
-    int32_t cb[4];
-    for (int j = 0; j < op.k; j++) cb[j] = op.bits[j] + N;
-    add_gate(op.k, op.bits, op.ctrl_mask, U);
-    add_gate(op.k, cb, op.ctrl_mask << N, Uc);
-  }
-  out->ntop = (int)out->ops.size();
-  out->chan.assign(out->ops.size(), bti::ChanDesc{1, 0, 0});  // no renormalisation bookkeeping
-  out->max_draws = 0;
-  out->has_measure = false;
-  return BT_OK;
 }
 
 }  // namespace bt
@@ -836,7 +410,7 @@ int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats) {
   return run_grid(rho, dm, 0, 0, nullptr, 0, nullptr, 0, &nout, stats, 0, tile_bits_for(rho->prec, rho->nbits));
 }
 
-int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes, int64_t outcap,
+int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes, int64_t outcap,
                  int64_t* nouts, double* final_states, bt_run_stats* stats) {
   if (!c) return fail(BT_ERR_ARG, "null circuit");
   if (ntraj <= 0 || !outcomes || !nouts || outcap < 0) return fail(BT_ERR_ARG, "bad batch arguments");
@@ -850,8 +424,8 @@ int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t n
   const uint64_t len = 1ull << nbits;
   const int T = tile_bits_for(prec, nbits);
   const int cus = ctx().cus > 0 ? ctx().cus : 256;
-  // one workgroup per CU for wide tiles (LDS-bound), two for narrow ones
-  int per_cu = (size_t)(1u << T) * elem > 80 * 1024 ? 1 : 2;
+  // workgroups resident per CU (LDS-bound: tile + tables)
+  int per_cu = 2;
   int64_t nwg = std::min<int64_t>(ntraj, (int64_t)cus * per_cu);
   // keep the per-workgroup states within a quarter of free HBM
   size_t freeb = 0, totalb = 0;
@@ -895,11 +469,11 @@ int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t n
       (final_states && hipMalloc((void**)&d_final, sizeof(double) * 2 * len * ntraj) != hipSuccess)) {
     cleanup();
     (void)hipGetLastError();
-    return fail(BT_ERR_NOMEM, "bt_run_batch: device allocation failed");
+    return fail(BT_ERR_NOMEM, "bt_run_batch_persistent: device allocation failed");
   }
   if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     cleanup();
-    return fail(BT_ERR_HIP, "bt_run_batch: stream/event");
+    return fail(BT_ERR_HIP, "bt_run_batch_persistent: stream/event");
   }
   (void)hipMemsetAsync(d_counter, 0, sizeof(unsigned long long), st);
   (void)hipMemsetAsync(d_stats, 0, sizeof(unsigned long long) * 8, st);
@@ -929,18 +503,17 @@ int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t n
   const size_t lds = (size_t)(1u << T) * elem;
   (void)hipEventRecord(e0, st);
   if (prec == BT_C64) {
-    (void)hipFuncSetAttribute((const void*)k_batch<float, kNT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_batch<float, kNT>), dim3((unsigned)nwg), dim3(kNT), lds, st, a);
+    (void)hipFuncSetAttribute((const void*)k_batch<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_batch<float>), dim3((unsigned)nwg), dim3(TileCfg<float>::NT), lds, st, a);
   } else {
-    (void)hipFuncSetAttribute((const void*)k_batch<double, kNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    hipLaunchKernelGGL((k_batch<double, kNT>), dim3((unsigned)nwg), dim3(kNT), lds, st, a);
+    (void)hipFuncSetAttribute((const void*)k_batch<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_batch<double>), dim3((unsigned)nwg), dim3(TileCfg<double>::NT), lds, st, a);
   }
   hipError_t le = hipGetLastError();
   (void)hipEventRecord(e1, st);
   hipError_t se = hipStreamSynchronize(st);
   if (le != hipSuccess || se != hipSuccess) {
-    std::string m = std::string("bt_run_batch: kernel failed: ") + hipGetErrorString(le != hipSuccess ? le : se);
+    std::string m = std::string("bt_run_batch_persistent: kernel failed: ") + hipGetErrorString(le != hipSuccess ? le : se);
     cleanup();
     return fail(BT_ERR_HIP, m);
   }
@@ -964,12 +537,13 @@ int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t n
     if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) stats->kernel_ms = ms;
   }
   cleanup();
-  if (herr) return fail(herr, "bt_run_batch: a trajectory failed (interpreter error " + std::to_string(herr) + ")");
+  if (herr) return fail(herr, "bt_run_batch_persistent: a trajectory failed (interpreter error " + std::to_string(herr) + ")");
   return BT_OK;
 }
 
 // Tile width override used by the autotuner (0 restores the default).
 int bt_set_tile_bits(int prec, int T) {
+  // (must not exceed log2(threads) + register bits of the tile pass)
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
   if (T != 0 && (T < 2 || T > default_tile_bits(prec))) return fail(BT_ERR_ARG, "tile bits out of range");
   g_tile_override[prec] = T;

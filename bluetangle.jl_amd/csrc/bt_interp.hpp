@@ -346,7 +346,7 @@ struct Interp {
 
   // choose the ops of the next pass for a candidate tile mask; returns count
   // matrix entries a pass can hold (the device keeps them in LDS)
-  static constexpr int kPassMatEntries = 1024;
+  static constexpr int kPassMatEntries = 768;
 
   BT_HD int select_ops(uint64_t tile, int32_t* sel) {
     uint64_t blocked = 0;
@@ -355,7 +355,10 @@ struct Interp {
       uint64_t b = maskof(Q(i).bits, Q(i).k);
       uint64_t touch = b | Q(i).ctrl;
       if ((touch & blocked) == 0 && (b & ~tile) == 0) {
-        const int e = 1 << (2 * Q(i).k);
+        // in-tile controls become operand bits of the pass's operator
+        int kc = Q(i).k + popc(Q(i).ctrl);
+        if (kc > 4) kc = 4;
+        const int e = 1 << (2 * kc);
         if (ents + e > kPassMatEntries) break;
         ents += e;
         sel[n++] = i;

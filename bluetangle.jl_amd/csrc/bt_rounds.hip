@@ -1,0 +1,577 @@
+// Lockstep ("rounds") executor: many states advance together.
+//
+// Each trajectory or request owns a device state and an interpreter that
+// runs on a host worker thread.  A round takes the next pass of every active
+// job, uploads the jobs' tile-local operator tables in one copy, and runs
+// them all in ONE kernel launch (k_round: one workgroup per group of tiles of
+// one job, states of any size and qubit count side by side), then one
+// finishing launch that reduces each job's FP64 partials in a fixed order.
+// Jobs leave as they finish and new ones are admitted into the freed HBM
+// (continuous batching); bt_run_batch and the server are both built on it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "bt_circuit.hpp"
+#include "bt_internal.hpp"
+#include "bt_interp.hpp"
+#include "bt_tile.hpp"
+
+namespace bt {
+
+int tile_bits_for(int prec, int nbits);
+
+// ---------------------------------------------------------------------------
+// device side
+// ---------------------------------------------------------------------------
+struct RoundItem {
+  void* state;
+  uint64_t table;     // byte offset of the job's table in the round's blob
+  uint32_t ntiles;    // tiles of the pass
+  uint32_t block0;    // first workgroup of this item
+  uint32_t nblocks;
+  int32_t fresh;      // state is |0..0> and not yet written
+};
+
+// table layout: PassGeom, then int32 nmats (padded to 16 B), then nmats matrix entries
+constexpr size_t kGeomBytes = (sizeof(PassGeom) + 15) / 16 * 16;
+
+template <typename R>
+__global__ void __launch_bounds__(TileCfg<R>::NT) k_round(const RoundItem* items, int nitems,
+                                                          const unsigned char* blob, int tpb, double* part) {
+  constexpr int NT = TileCfg<R>::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<R>* tile = (C<R>*)smem;
+  __shared__ PassGeom geom;
+  __shared__ C<R> mats[kMatEntries];
+  __shared__ double red[(NT / 64) * 17];
+  __shared__ double sums[kAcc];
+  __shared__ int item_sh;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].block0 <= blockIdx.x) lo = mid;
+      else hi = mid - 1;
+    }
+    item_sh = lo;
+  }
+  __syncthreads();
+  const RoundItem it = items[item_sh];
+  const unsigned char* tab = blob + it.table;
+  {
+    const int* src = (const int*)tab;
+    int* dst = (int*)&geom;
+    for (int i = threadIdx.x; i < (int)(sizeof(PassGeom) / sizeof(int)); i += NT) dst[i] = src[i];
+    const int nm = *(const int*)(tab + kGeomBytes);
+    const C<R>* m = (const C<R>*)(tab + kGeomBytes + 16);
+    for (int i = threadIdx.x; i < nm; i += NT) mats[i] = m[i];
+  }
+  if (threadIdx.x < kAcc) sums[threadIdx.x] = 0.0;
+  __syncthreads();
+  const uint32_t local = blockIdx.x - it.block0;
+  const uint64_t t0 = (uint64_t)local * tpb;
+  const uint64_t tend = std::min<uint64_t>((uint64_t)it.ntiles, t0 + tpb);
+  tile_pass<R>((C<R>*)it.state, geom, mats, tile, t0, 1, tend, red, sums, it.fresh);
+  if (threadIdx.x < kAcc) part[(uint64_t)blockIdx.x * kAcc + threadIdx.x] = sums[threadIdx.x];
+}
+
+__global__ void k_round_finalize(const RoundItem* items, const double* part, double* out) {
+  const RoundItem it = items[blockIdx.x];
+  for (int j = threadIdx.x; j < kAcc; j += blockDim.x) {
+    double t = 0.0;
+    for (uint32_t b = 0; b < it.nblocks; b++) t += part[(uint64_t)(it.block0 + b) * kAcc + j];
+    out[(uint64_t)blockIdx.x * kAcc + j] = t;
+  }
+}
+
+template <typename R>
+__global__ void k_export_state(const C<R>* s, uint64_t n, double* out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    C<R> v = s[i];
+    out[2 * i] = (double)v.x;
+    out[2 * i + 1] = (double)v.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host worker pool
+// ---------------------------------------------------------------------------
+class Pool {
+ public:
+  explicit Pool(int n) {
+    for (int i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+      gen_++;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  // run f(i) for i in [0, n) on the pool and the caller; returns when done
+  void parallel_for(int n, const std::function<void(int)>& f) {
+    if (n <= 0) return;
+    if (th_.empty() || n == 1) {
+      for (int i = 0; i < n; i++) f(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      fn_ = &f;
+      n_ = n;
+      next_.store(0);
+      busy_ = (int)th_.size();
+      gen_++;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [this] { return busy_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void work() {
+    for (;;) {
+      int i = next_.fetch_add(1);
+      if (i >= n_) break;
+      (*fn_)(i);
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+      }
+      work();
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        busy_--;
+      }
+      done_cv_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)>* fn_ = nullptr;
+  std::atomic<int> next_{0};
+  int n_ = 0, busy_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+static int pool_threads() {
+  unsigned h = std::thread::hardware_concurrency();
+  int n = h ? (int)h : 8;
+  return std::max(1, std::min(16, n) - 1);
+}
+
+// ---------------------------------------------------------------------------
+// jobs and rounds
+// ---------------------------------------------------------------------------
+struct Job {
+  int64_t id = 0;
+  std::shared_ptr<const btc::CircuitHost> circ;
+  int nbits = 0;
+  int prec = BT_C64;
+  int fold = 1;
+  uint64_t seed = 0, traj = 0;
+  void* state = nullptr;  // device
+  size_t bytes = 0;
+  std::unique_ptr<bti::Interp> ip;
+  std::vector<QOp> slots;
+  Plan plan;
+  PassResult res;
+  int has_plan = 0, fresh = 1, done = 0, err = 0, red_swap = 0;
+  std::vector<int32_t> outs;
+  std::vector<unsigned char> table;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  int64_t passes = 0, rpasses = 0, boundary = 0, draws = 0;
+  double bytes_moved = 0;
+};
+
+static double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+class RoundEngine {
+ public:
+  explicit RoundEngine(int prec) : prec_(prec), pool_(pool_threads()) {}
+  ~RoundEngine() { release(); }
+
+  int init() {
+    int rc = ensure_device();
+    if (rc) return rc;
+    BT_HIP(hipStreamCreate(&stream_));
+    BT_HIP(hipEventCreate(&ev_a_));
+    BT_HIP(hipEventCreate(&ev_b_));
+    return BT_OK;
+  }
+
+  void release() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (h_blob_) (void)hipHostFree(h_blob_);
+    if (d_blob_) (void)hipFree(d_blob_);
+    if (h_items_) (void)hipHostFree(h_items_);
+    if (d_items_) (void)hipFree(d_items_);
+    if (d_part_) (void)hipFree(d_part_);
+    if (d_out_) (void)hipFree(d_out_);
+    if (h_out_) (void)hipHostFree(h_out_);
+    if (ev_a_) (void)hipEventDestroy(ev_a_);
+    if (ev_b_) (void)hipEventDestroy(ev_b_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    ev_a_ = ev_b_ = nullptr;
+    h_blob_ = d_blob_ = nullptr;
+    h_items_ = nullptr;
+    d_items_ = nullptr;
+    d_part_ = d_out_ = h_out_ = nullptr;
+    stream_ = nullptr;
+  }
+
+  hipStream_t stream() const { return stream_; }
+  int tile_T(int nbits) const { return tile_bits_for(prec_, nbits); }
+  size_t elem() const { return prec_ == BT_C64 ? 8 : 16; }
+
+  void start_job(Job* j) {
+    j->ip.reset(new bti::Interp());
+    j->slots.assign(bti::kQMax, QOp());
+    bti::Program prog{j->circ->ops.data(), j->circ->ntop, j->circ->pool.data(), j->circ->chan.data(),
+                      j->circ->chanw.data(), j->nbits};
+    j->ip->init(prog, j->slots.data(), tile_T(j->nbits), 3);
+    j->ip->seed = j->seed;
+    j->ip->traj = j->traj;
+    j->ip->fold = j->fold;
+    j->ip->tol_exact = prec_ == BT_C64 ? 1e-6 : 1e-12;
+    j->outs.assign(std::max<size_t>(16, 2 * j->circ->ops.size()), 0);
+    j->ip->outs = j->outs.data();
+    j->ip->outcap = (int64_t)j->outs.size();
+    j->has_plan = 0;
+    j->fresh = 1;
+    j->done = 0;
+    j->err = 0;
+    memset(&j->res, 0, sizeof j->res);
+  }
+
+  // One round over `jobs` (all unfinished).  Returns BT_OK or an error.
+  int round(std::vector<Job*>& jobs) {
+    // 1. advance every interpreter to its next pass and build its table
+    pool_.parallel_for((int)jobs.size(), [&](int i) {
+      Job* j = jobs[i];
+      if (j->done) return;
+      const int c = j->ip->step(&j->plan, &j->res);
+      if (j->ip->err) {
+        j->err = j->ip->err;
+        j->done = 1;
+        j->has_plan = 0;
+        return;
+      }
+      if (!c) {
+        j->done = 1;
+        j->has_plan = 0;
+        return;
+      }
+      j->has_plan = 1;
+      const size_t need = kGeomBytes + 16 + sizeof(C<double>) * kMatEntries;
+      j->table.resize(need);
+      PassGeom* g = (PassGeom*)j->table.data();
+      int nm = 0, grc;
+      if (prec_ == BT_C64) {
+        C<float>* m = (C<float>*)(j->table.data() + kGeomBytes + 16);
+        grc = build_geom<float>(j->plan, j->slots.data(), j->nbits, g, m);
+      } else {
+        C<double>* m = (C<double>*)(j->table.data() + kGeomBytes + 16);
+        grc = build_geom<double>(j->plan, j->slots.data(), j->nbits, g, m);
+      }
+      if (grc) {
+        j->err = grc;
+        j->done = 1;
+        j->has_plan = 0;
+        return;
+      }
+      j->red_swap = g->red_swap;
+      nm = g->nmats;
+      *(int*)(j->table.data() + kGeomBytes) = nm;
+      j->table.resize(kGeomBytes + 16 + elem() * nm);
+    });
+    // 2. pack
+    std::vector<Job*> act;
+    for (Job* j : jobs)
+      if (j->has_plan) act.push_back(j);
+    if (act.empty()) return BT_OK;
+    const int tpb = 2;
+    size_t blob = 0;
+    uint32_t blocks = 0;
+    std::vector<RoundItem> items(act.size());
+    size_t maxT = 0;
+    for (size_t i = 0; i < act.size(); i++) {
+      Job* j = act[i];
+      const int T = j->plan.ntile;
+      maxT = std::max<size_t>(maxT, (size_t)T);
+      RoundItem& it = items[i];
+      it.state = j->state;
+      it.table = blob;
+      it.ntiles = (uint32_t)(1ull << (j->nbits - T));
+      it.block0 = blocks;
+      it.nblocks = (it.ntiles + tpb - 1) / tpb;
+      it.fresh = j->fresh;
+      blocks += it.nblocks;
+      blob += (j->table.size() + 15) / 16 * 16;
+    }
+    int rc = reserve(blob, act.size(), blocks);
+    if (rc) return rc;
+    pool_.parallel_for((int)act.size(), [&](int i) {
+      memcpy((unsigned char*)h_blob_ + items[i].table, act[i]->table.data(), act[i]->table.size());
+    });
+    memcpy(h_items_, items.data(), sizeof(RoundItem) * items.size());
+    BT_HIP(hipMemcpyAsync(d_blob_, h_blob_, blob, hipMemcpyHostToDevice, stream_));
+    BT_HIP(hipMemcpyAsync(d_items_, h_items_, sizeof(RoundItem) * items.size(), hipMemcpyHostToDevice, stream_));
+    // 3. launch
+    const size_t lds = (size_t)(1ull << maxT) * elem();
+    BT_HIP(hipEventRecord(ev_a_, stream_));
+    if (prec_ == BT_C64) {
+      (void)hipFuncSetAttribute((const void*)k_round<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_round<float>), dim3(blocks), dim3(TileCfg<float>::NT), lds, stream_, d_items_,
+                         (int)items.size(), (const unsigned char*)d_blob_, tpb, d_part_);
+    } else {
+      (void)hipFuncSetAttribute((const void*)k_round<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_round<double>), dim3(blocks), dim3(TileCfg<double>::NT), lds, stream_, d_items_,
+                         (int)items.size(), (const unsigned char*)d_blob_, tpb, d_part_);
+    }
+    BT_HIP(hipGetLastError());
+    BT_HIP(hipEventRecord(ev_b_, stream_));
+    hipLaunchKernelGGL(k_round_finalize, dim3((unsigned)items.size()), dim3(64), 0, stream_, d_items_, d_part_,
+                       d_out_);
+    BT_HIP(hipGetLastError());
+    BT_HIP(hipMemcpyAsync(h_out_, d_out_, sizeof(double) * kAcc * items.size(), hipMemcpyDeviceToHost, stream_));
+    BT_HIP(hipStreamSynchronize(stream_));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ev_a_, ev_b_) == hipSuccess) pass_ms_ += ms;
+    rounds_++;
+    launched_blocks_ += blocks;
+    // 4. hand the results back
+    for (size_t i = 0; i < act.size(); i++) {
+      Job* j = act[i];
+      to_result(j->plan, j->red_swap, h_out_ + kAcc * i, &j->res);
+      if (j->plan.write) j->fresh = 0;
+    }
+    return BT_OK;
+  }
+
+  // collect a finished job's statistics
+  static void finish_stats(Job* j, size_t elem) {
+    j->passes = j->ip->passes;
+    j->rpasses = j->ip->rpasses;
+    j->boundary = j->ip->boundary;
+    j->draws = j->ip->draw;
+    // read+write per pass, read per reduction pass
+    j->bytes_moved = j->ip->bytes_units * (double)(1ull << j->nbits) * (double)elem;
+  }
+
+  int64_t rounds() const { return rounds_; }
+  double pass_ms() const { return pass_ms_; }
+
+ private:
+  int reserve(size_t blob, size_t nitems, uint32_t blocks) {
+    if (blob > cap_blob_) {
+      if (h_blob_) (void)hipHostFree(h_blob_);
+      if (d_blob_) (void)hipFree(d_blob_);
+      h_blob_ = d_blob_ = nullptr;
+      cap_blob_ = std::max(blob, cap_blob_ * 2);
+      BT_HIP(hipHostMalloc(&h_blob_, cap_blob_));
+      BT_HIP(hipMalloc(&d_blob_, cap_blob_));
+    }
+    if (nitems > cap_items_) {
+      if (h_items_) (void)hipHostFree(h_items_);
+      if (d_items_) (void)hipFree(d_items_);
+      if (d_out_) (void)hipFree(d_out_);
+      if (h_out_) (void)hipHostFree(h_out_);
+      cap_items_ = std::max(nitems, cap_items_ * 2);
+      BT_HIP(hipHostMalloc((void**)&h_items_, sizeof(RoundItem) * cap_items_));
+      BT_HIP(hipMalloc((void**)&d_items_, sizeof(RoundItem) * cap_items_));
+      BT_HIP(hipMalloc((void**)&d_out_, sizeof(double) * kAcc * cap_items_));
+      BT_HIP(hipHostMalloc((void**)&h_out_, sizeof(double) * kAcc * cap_items_));
+    }
+    if (blocks > cap_blocks_) {
+      if (d_part_) (void)hipFree(d_part_);
+      cap_blocks_ = std::max<size_t>(blocks, cap_blocks_ * 2);
+      BT_HIP(hipMalloc((void**)&d_part_, sizeof(double) * kAcc * cap_blocks_));
+    }
+    return BT_OK;
+  }
+
+  int prec_;
+  Pool pool_;
+  hipStream_t stream_ = nullptr;
+  void* h_blob_ = nullptr;
+  void* d_blob_ = nullptr;
+  size_t cap_blob_ = 0;
+  RoundItem* h_items_ = nullptr;
+  RoundItem* d_items_ = nullptr;
+  size_t cap_items_ = 0;
+  double* d_part_ = nullptr;
+  size_t cap_blocks_ = 0;
+  double* d_out_ = nullptr;
+  double* h_out_ = nullptr;
+  int64_t rounds_ = 0;
+  uint64_t launched_blocks_ = 0;
+  hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
+  double pass_ms_ = 0;
+};
+
+// Export a finished job's state as complex128 into host memory.
+static int export_state(Job* j, hipStream_t st, double* host, double* d_tmp) {
+  const uint64_t n = 1ull << j->nbits;
+  if (j->fresh) {  // never written: still |0..0>
+    memset(host, 0, sizeof(double) * 2 * n);
+    host[0] = 1.0;
+    return BT_OK;
+  }
+  const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 16);
+  if (j->prec == BT_C64)
+    hipLaunchKernelGGL(k_export_state<float>, dim3(grid), dim3(256), 0, st, (const C<float>*)j->state, n, d_tmp);
+  else
+    hipLaunchKernelGGL(k_export_state<double>, dim3(grid), dim3(256), 0, st, (const C<double>*)j->state, n, d_tmp);
+  BT_HIP(hipGetLastError());
+  BT_HIP(hipMemcpyAsync(host, d_tmp, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+  BT_HIP(hipStreamSynchronize(st));
+  return BT_OK;
+}
+
+}  // namespace bt
+
+using namespace bt;
+
+extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
+                            int64_t outcap, int64_t* nouts, double* final_states, bt_run_stats* stats) {
+  if (!c) return fail(BT_ERR_ARG, "null circuit");
+  if (ntraj <= 0 || !outcomes || !nouts || outcap < 0) return fail(BT_ERR_ARG, "bad batch arguments");
+  if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
+  RoundEngine eng(prec);
+  int rc = eng.init();
+  if (rc) return rc;
+  auto circ = std::shared_ptr<const btc::CircuitHost>(&c->h, [](const btc::CircuitHost*) {});
+  const int nbits = c->h.n;
+  const size_t bytes = (size_t)(1ull << nbits) * eng.elem();
+  size_t freeb = 0, totalb = 0;
+  BT_HIP(hipMemGetInfo(&freeb, &totalb));
+  // resident states: all of them when they fit in 3/4 of free HBM
+  int64_t nslots = std::min<int64_t>(ntraj, (int64_t)(freeb / 4 * 3 / (bytes + 4096)));
+  if (nslots < 1) return fail(BT_ERR_NOMEM, "bt_run_batch: a single state does not fit in HBM");
+  std::vector<void*> slot_mem;
+  double* d_tmp = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : slot_mem) (void)hipFree(p);
+    if (d_tmp) (void)hipFree(d_tmp);
+  };
+  {
+    // one allocation for all slots
+    void* p = nullptr;
+    while (nslots >= 1 && hipMalloc(&p, bytes * nslots) != hipSuccess) {
+      (void)hipGetLastError();
+      nslots /= 2;
+    }
+    if (nslots < 1) return fail(BT_ERR_NOMEM, "bt_run_batch: state allocation failed");
+    slot_mem.push_back(p);
+  }
+  if (final_states && hipMalloc((void**)&d_tmp, sizeof(double) * 2 * (1ull << nbits)) != hipSuccess) {
+    cleanup();
+    return fail(BT_ERR_NOMEM, "bt_run_batch: export buffer");
+  }
+  std::vector<std::unique_ptr<Job>> jobs(ntraj);
+  std::vector<int64_t> free_slots;
+  for (int64_t s = nslots - 1; s >= 0; s--) free_slots.push_back(s);
+  std::vector<Job*> active;
+  std::vector<int64_t> slot_of(ntraj, -1);
+  int64_t next = 0, done = 0;
+  hipEvent_t e0, e1;
+  BT_HIP(hipEventCreate(&e0));
+  BT_HIP(hipEventCreate(&e1));
+  (void)hipEventRecord(e0, eng.stream());
+  bt_run_stats acc;
+  memset(&acc, 0, sizeof acc);
+  int first_err = 0;
+  while (done < ntraj) {
+    while (next < ntraj && !free_slots.empty()) {
+      const int64_t s = free_slots.back();
+      free_slots.pop_back();
+      jobs[next].reset(new Job());
+      Job* j = jobs[next].get();
+      j->id = next;
+      j->circ = circ;
+      j->nbits = nbits;
+      j->prec = prec;
+      j->seed = seed;
+      j->traj = (uint64_t)(traj0 + next);
+      j->state = (char*)slot_mem[0] + bytes * s;
+      j->bytes = bytes;
+      slot_of[next] = s;
+      eng.start_job(j);
+      active.push_back(j);
+      next++;
+    }
+    rc = eng.round(active);
+    if (rc) break;
+    std::vector<Job*> still;
+    for (Job* j : active) {
+      if (!j->done) {
+        still.push_back(j);
+        continue;
+      }
+      RoundEngine::finish_stats(j, eng.elem());
+      acc.passes += j->passes;
+      acc.reduce_passes += j->rpasses;
+      acc.boundary_draws += j->boundary;
+      acc.draws += j->draws;
+      acc.bytes += j->bytes_moved;
+      if (j->err && !first_err) first_err = j->err;
+      const int64_t t = j->id;
+      const int64_t n = std::min<int64_t>(outcap, j->ip->nout);
+      for (int64_t i = 0; i < n; i++) outcomes[t * outcap + i] = j->outs[i];
+      nouts[t] = j->ip->nout;
+      if (final_states) {
+        rc = export_state(j, eng.stream(), final_states + (uint64_t)t * 2 * (1ull << nbits), d_tmp);
+        if (rc) break;
+      }
+      free_slots.push_back(slot_of[t]);
+      jobs[t].reset();
+      done++;
+    }
+    if (rc) break;
+    active.swap(still);
+  }
+  (void)hipEventRecord(e1, eng.stream());
+  (void)hipStreamSynchronize(eng.stream());
+  // device time of the pass kernels (HIP events around each launch)
+  acc.kernel_ms = eng.pass_ms();
+  if (stats) *stats = acc;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  cleanup();
+  if (rc) return rc;
+  if (first_err) return fail(first_err, "bt_run_batch: a trajectory failed (interpreter error " +
+                                            std::to_string(first_err) + ")");
+  return BT_OK;
+}

@@ -1,0 +1,750 @@
+// Tile pass shared by the circuit executors.
+//
+// A pass streams the state through the chip in tiles of 2^T amplitudes whose
+// T local bits are the pass's tile bits.  A workgroup of NT threads keeps a
+// tile in registers, 2^RB = 32 amplitudes per thread: RB "register bits" of
+// the local index are held inside each thread and the log2(NT) others select
+// the thread.  Operators are applied in registers with the matrix pattern
+// fixed at compile time (one instantiation per set of register positions),
+// so a pass with many fused operators costs FMAs, not memory traffic.  When
+// the next operators need other bits in registers the tile is re-laid out
+// through LDS (one swizzled write + read: a "phase" change).  Global
+// addresses are bit deposits of (tile index, thread, register) - no tables -
+// and the layouts used to load and store keep the low local bits on the
+// threads, so each wavefront touches contiguous 64 B+ runs.  Reductions (norm,
+// histogram over up to 4 bits, reduced rho over up to 2 bits) run in FP64
+// and are added per tile in a fixed order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bt_internal.hpp"
+#include "bt_interp.hpp"
+
+using bti::Plan;
+using bti::PassResult;
+using bti::QOp;
+
+namespace bt {
+
+constexpr int kMatEntries = bti::Interp::kPassMatEntries;
+constexpr int kAcc = 33;   // norm2, hist[16], packed rho[16]
+constexpr int kRB = 5;     // register bits (most): 32 amplitudes per thread
+constexpr int kMaxPh = 28;
+
+// 256 threads per workgroup; complex64 keeps 32 amplitudes per thread (T =
+// 13), complex128 16 (T = 12): 64 KiB tiles, registers <= 256 per lane.
+template <typename R>
+struct TileCfg;
+template <>
+struct TileCfg<float> {
+  static constexpr int NT = 256;
+  static constexpr int LOGNT = 8;
+  static constexpr int RB = 5;
+};
+template <>
+struct TileCfg<double> {
+  static constexpr int NT = 256;
+  static constexpr int LOGNT = 8;
+  static constexpr int RB = 4;
+};
+
+// An operator of a pass.  k <= 2: applied in registers, `mask` = register
+// positions, matrix index bit j <-> j-th lowest position.  k >= 3 (rare:
+// 3-bit channels, doubly controlled gates, 2-qubit density-matrix
+// superoperators): applied to the tile in LDS, matrix in operand order
+// (operand 0 = MSB) at local offsets loff.
+struct ROp {
+  int32_t k;
+  int32_t mask;
+  int32_t moff;
+  int32_t lsorted[4];  // k >= 3: ascending local bits
+  uint32_t loff[16];   // k >= 3: local offset of each matrix index
+  uint64_t gctrl;      // control bits outside the tile (global positions)
+};
+
+struct RPhase {
+  int8_t rb[kRB];  // local tile bit held at register position j (-1: phantom)
+  int8_t pad[3];
+  int32_t op0, nops;
+};
+
+struct PassGeom {
+  int32_t T, nrb, tb, ncomp, write, need_norm, red, red_k, red_swap, nph, nops, nmats;
+  int32_t tile[bti::kMaxT];  // local bit -> global bit
+  int32_t comp[40];          // global bits outside the tile, ascending
+  int32_t nrun;              // tile index -> global base: runs of consecutive outside bits
+  int8_t run_dst[bti::kMaxT + 1], run_len[bti::kMaxT + 1];
+  int32_t red_bits[4];       // RED_HIST: global bits; RED_RHO: register positions (last phase), ascending
+  int8_t io_in[kRB], io_out[kRB];
+  int8_t pad[6];
+  double scale;
+  RPhase ph[kMaxPh];
+  ROp ops[bti::kPMax];
+};
+
+// ---------------------------------------------------------------------------
+// host/device: plan + queue slots -> register-tile geometry and matrices
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int popc32(uint32_t x) {
+  int c = 0;
+  while (x) {
+    x &= x - 1;
+    c++;
+  }
+  return c;
+}
+
+// Layout with register positions 0..nrb-1 holding the given local bits
+// (ascending) and the rest phantom.
+__host__ __device__ inline void set_layout(int8_t* rb, const int* bits, int n) {
+  for (int j = 0; j < kRB; j++) rb[j] = j < n ? (int8_t)bits[j] : (int8_t)-1;
+}
+
+template <typename R>
+__host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, const QOp* slots, int nbits, PassGeom* g,
+                                                             C<R>* mats) {
+  const int T = plan.ntile;
+  g->T = T;
+  constexpr int RB = TileCfg<R>::RB;
+  g->nrb = T < RB ? T : RB;
+  g->tb = T - g->nrb;
+  for (int i = 0; i < T; i++) g->tile[i] = plan.tile[i];
+  uint64_t tmask = 0;
+  for (int i = 0; i < T; i++) tmask |= 1ull << plan.tile[i];
+  g->ncomp = 0;
+  for (int b = 0; b < nbits; b++)
+    if (!((tmask >> b) & 1)) g->comp[g->ncomp++] = b;
+  g->nrun = 0;
+  for (int i = 0; i < g->ncomp; i++) {
+    if (i == 0 || g->comp[i] != g->comp[i - 1] + 1) {
+      if (g->nrun > bti::kMaxT) return BT_ERR_UNSUPPORTED;
+      g->run_dst[g->nrun] = (int8_t)g->comp[i];
+      g->run_len[g->nrun] = 0;
+      g->nrun++;
+    }
+    g->run_len[g->nrun - 1]++;
+  }
+  int rank[64];
+  for (int b = 0; b < 64; b++) rank[b] = -1;
+  for (int i = 0; i < T; i++) rank[plan.tile[i]] = i;
+  g->write = plan.write;
+  g->need_norm = (plan.aligned || plan.red != bti::RED_NONE) ? 1 : 0;
+  g->scale = plan.scale;
+  g->red = plan.red;
+  g->red_k = plan.red_k;
+  g->red_swap = 0;
+  const int nrb = g->nrb;
+  // 1. operators in local bits; in-tile controls become target bits
+  int lbits[bti::kPMax][kRB];
+  int lk[bti::kPMax];
+  for (int o = 0; o < plan.nops; o++) {
+    const QOp& q = slots[plan.qidx[o]];
+    int nb = 0;
+    for (int b = 0; b < nbits; b++)
+      if (((q.ctrl >> b) & 1) && rank[b] >= 0) {
+        if (nb >= kRB) return BT_ERR_UNSUPPORTED;
+        lbits[o][nb++] = rank[b];
+      }
+    for (int i = 0; i < q.k; i++) {
+      if (nb >= kRB) return BT_ERR_UNSUPPORTED;
+      lbits[o][nb++] = rank[q.bits[i]];
+    }
+    if (nb > 4) return BT_ERR_UNSUPPORTED;
+    lk[o] = nb;
+  }
+  // 2. phases: consecutive operators whose bits fit in the registers
+  int nph = 0;
+  uint32_t cur = 0;  // local-bit mask of the open phase
+  int ph_of[bti::kPMax];
+  uint32_t ph_mask[kMaxPh];
+  for (int o = 0; o < plan.nops; o++) {
+    uint32_t m = 0;
+    if (lk[o] <= 2)
+      for (int i = 0; i < lk[o]; i++) m |= 1u << lbits[o][i];
+    if (nph == 0 || popc32(cur | m) > nrb) {
+      if (nph >= kMaxPh) return BT_ERR_UNSUPPORTED;
+      ph_mask[nph] = 0;
+      nph++;
+      cur = 0;
+    }
+    cur |= m;
+    ph_mask[nph - 1] = cur;
+    ph_of[o] = nph - 1;
+  }
+  uint32_t redm = 0;
+  if (plan.red == bti::RED_RHO) {
+    for (int i = 0; i < plan.red_k; i++) redm |= 1u << rank[plan.red_bits[i]];
+    if (nph == 0 || popc32(cur | redm) > nrb) {
+      if (nph >= kMaxPh) return BT_ERR_UNSUPPORTED;
+      ph_mask[nph] = 0;
+      nph++;
+      cur = 0;
+    }
+    cur |= redm;
+    ph_mask[nph - 1] = cur;
+  }
+  // 3. fill every phase to nrb register bits with the highest free local bits
+  //    (the low bits stay on the threads), positions = ascending local bits
+  for (int p = 0; p < nph; p++) {
+    uint32_t m = ph_mask[p];
+    for (int b = T - 1; b >= 0 && popc32(m) < nrb; b--) m |= 1u << b;
+    ph_mask[p] = m;
+    int bits[kRB], n = 0;
+    for (int b = 0; b < T; b++)
+      if ((m >> b) & 1) bits[n++] = b;
+    set_layout(g->ph[p].rb, bits, n);
+    g->ph[p].op0 = 0;
+    g->ph[p].nops = 0;
+  }
+  g->nph = nph;
+  // load/store layouts: the top local bits in registers, unless a phase's own
+  // layout keeps the lowest min(3, tb) local bits on the threads
+  {
+    int bits[kRB], n = 0;
+    for (int b = T - nrb; b < T; b++) bits[n++] = b;
+    set_layout(g->io_in, bits, n);
+    set_layout(g->io_out, bits, n);
+    const uint32_t low = (1u << (g->tb < 3 ? g->tb : 3)) - 1u;
+    if (nph > 0 && !(ph_mask[0] & low))
+      for (int j = 0; j < kRB; j++) g->io_in[j] = g->ph[0].rb[j];
+    if (nph > 0 && !(ph_mask[nph - 1] & low))
+      for (int j = 0; j < kRB; j++) g->io_out[j] = g->ph[nph - 1].rb[j];
+  }
+  // 4. operators in register positions, matrices permuted to the ascending
+  //    position convention (index bit j <-> j-th lowest position)
+  int moff = 0;
+  g->nops = plan.nops;
+  for (int o = 0; o < plan.nops; o++) {
+    const QOp& q = slots[plan.qidx[o]];
+    const RPhase& ph = g->ph[ph_of[o]];
+    if (ph.nops == 0) g->ph[ph_of[o]].op0 = o;
+    g->ph[ph_of[o]].nops++;
+    ROp& r = g->ops[o];
+    const int k2 = lk[o], D2 = 1 << k2;
+    r.k = k2;
+    int pos[kRB];  // register position of lbits[o][i] (k <= 2), else the operand order
+    r.mask = 0;
+    if (k2 <= 2) {
+      for (int i = 0; i < k2; i++)
+        for (int j = 0; j < kRB; j++)
+          if (ph.rb[j] == lbits[o][i]) pos[i] = j;
+      for (int i = 0; i < k2; i++) r.mask |= 1 << pos[i];
+    } else {
+      for (int i = 0; i < k2; i++) pos[i] = k2 - 1 - i;  // identity: operand order, operand 0 = MSB
+      for (int i = 0; i < k2; i++) r.lsorted[i] = lbits[o][i];
+      for (int i = 0; i < k2; i++)
+        for (int j = i + 1; j < k2; j++)
+          if (r.lsorted[j] < r.lsorted[i]) {
+            int t = r.lsorted[i];
+            r.lsorted[i] = r.lsorted[j];
+            r.lsorted[j] = t;
+          }
+      for (int l = 0; l < D2; l++) {
+        uint32_t off = 0;
+        for (int i = 0; i < k2; i++)
+          if ((l >> (k2 - 1 - i)) & 1) off |= 1u << lbits[o][i];
+        r.loff[l] = off;
+      }
+    }
+    r.gctrl = 0;
+    for (int b = 0; b < nbits; b++)
+      if (((q.ctrl >> b) & 1) && rank[b] < 0) r.gctrl |= 1ull << b;
+    // rank of each operand bit among the ascending positions
+    int asc[kRB];
+    for (int i = 0; i < k2; i++) {
+      int c = 0;
+      for (int j = 0; j < k2; j++) c += pos[j] < pos[i];
+      asc[i] = c;
+    }
+    const int nc = k2 - q.k;  // leading operand bits are in-tile controls
+    const int D = 1 << q.k;
+    if (moff + D2 * D2 > kMatEntries) return BT_ERR_UNSUPPORTED;
+    r.moff = moff;
+    for (int nr = 0; nr < D2; nr++)
+      for (int ncl = 0; ncl < D2; ncl++) {
+        // operand-order indices (operand 0 = MSB) of the new indices
+        int orr = 0, occ = 0;
+        for (int i = 0; i < k2; i++) {
+          orr |= ((nr >> asc[i]) & 1) << (k2 - 1 - i);
+          occ |= ((ncl >> asc[i]) & 1) << (k2 - 1 - i);
+        }
+        const int cr = orr >> q.k, cc = occ >> q.k;
+        const int tr = orr & (D - 1), tc = occ & (D - 1);
+        double vr = 0, vi = 0;
+        if (cr == cc) {
+          if (cr == (1 << nc) - 1) {
+            vr = q.m[2 * (tr + tc * D)];
+            vi = q.m[2 * (tr + tc * D) + 1];
+          } else if (tr == tc) {
+            vr = 1.0;
+          }
+        }
+        mats[moff + nr + ncl * D2].x = (R)vr;
+        mats[moff + nr + ncl * D2].y = (R)vi;
+      }
+    moff += D2 * D2;
+  }
+  g->nmats = moff;
+  // 5. reductions
+  if (plan.red == bti::RED_HIST) {
+    for (int i = 0; i < plan.red_k; i++) g->red_bits[i] = plan.red_bits[i];
+  } else if (plan.red == bti::RED_RHO) {
+    const RPhase& ph = g->ph[nph - 1];
+    int p[2] = {0, 0};
+    for (int i = 0; i < plan.red_k; i++)
+      for (int j = 0; j < kRB; j++)
+        if (ph.rb[j] == rank[plan.red_bits[i]]) p[i] = j;
+    if (plan.red_k == 1) {
+      g->red_bits[0] = p[0];
+    } else {
+      g->red_bits[0] = p[0] < p[1] ? p[0] : p[1];
+      g->red_bits[1] = p[0] < p[1] ? p[1] : p[0];
+      // packed rho is in ascending-position index order (bit 0 = lower
+      // position); the plan wants red_bits[0] as the MSB
+      g->red_swap = p[0] < p[1] ? 1 : 0;
+    }
+  }
+  return BT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device side
+// ---------------------------------------------------------------------------
+template <typename R, int MASK>
+__host__ __device__ __forceinline__ void apply_reg(C<R>* v, const C<R>* mats) {
+  constexpr int K = __builtin_popcount(MASK);
+  constexpr int D = 1 << K;
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  int P[K > 0 ? K : 1];
+  {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kRB; j++)
+      if ((MASK >> j) & 1) P[c++] = j;
+  }
+#pragma unroll
+  for (int base = 0; base < NR; base++) {
+    if (base & MASK) continue;
+    C<R> x[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+      int idx = base;
+#pragma unroll
+      for (int j = 0; j < K; j++)
+        if ((l >> j) & 1) idx |= 1 << P[j];
+      x[l] = v[idx];
+    }
+#pragma unroll
+    for (int r = 0; r < D; r++) {
+      C<R> acc = cmake<C<R>>(0, 0);
+#pragma unroll
+      for (int c = 0; c < D; c++) acc = cfma(acc, mats[r + c * D], x[c]);
+      int idx = base;
+#pragma unroll
+      for (int j = 0; j < K; j++)
+        if ((r >> j) & 1) idx |= 1 << P[j];
+      v[idx] = acc;
+    }
+  }
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void apply_mask(int mask, C<R>* v, const C<R>* m) {
+  switch (mask) {
+#define BT_CASE(M) \
+  case M: apply_reg<R, M>(v, m); break;
+#define BT_CASE5(M)                                   \
+  case M:                                             \
+    if constexpr (TileCfg<R>::RB == 5) apply_reg<R, M>(v, m); \
+    break;
+    BT_CASE(1) BT_CASE(2) BT_CASE(4) BT_CASE(8) BT_CASE(3) BT_CASE(5) BT_CASE(6) BT_CASE(9) BT_CASE(10) BT_CASE(12)
+    BT_CASE5(16) BT_CASE5(17) BT_CASE5(18) BT_CASE5(20) BT_CASE5(24)
+#undef BT_CASE
+#undef BT_CASE5
+    default: break;
+  }
+}
+
+// reduced rho over register positions P0 < P1 (K = 1 or 2), packed as
+// diagonal then (a < b) Re, Im of rho[a, b]; index bit j <-> P_j
+template <typename R, int MASK>
+__host__ __device__ __forceinline__ void rho_reg(const C<R>* v, int nvalid_mask, double* racc) {
+  constexpr int K = __builtin_popcount(MASK);
+  constexpr int D = 1 << K;
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  int P[K];
+  {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kRB; j++)
+      if ((MASK >> j) & 1) P[c++] = j;
+  }
+#pragma unroll
+  for (int base = 0; base < NR; base++) {
+    if (base & MASK) continue;
+    if (base & nvalid_mask) continue;  // phantom registers
+    double vr[D], vi[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+      int idx = base;
+#pragma unroll
+      for (int j = 0; j < K; j++)
+        if ((l >> j) & 1) idx |= 1 << P[j];
+      vr[l] = (double)v[idx].x;
+      vi[l] = (double)v[idx].y;
+    }
+    int p = D;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      racc[a] += vr[a] * vr[a] + vi[a] * vi[a];
+#pragma unroll
+      for (int b = a + 1; b < D; b++) {
+        racc[p] += vr[a] * vr[b] + vi[a] * vi[b];
+        racc[p + 1] += vi[a] * vr[b] - vr[a] * vi[b];
+        p += 2;
+      }
+    }
+  }
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void rho_mask(int mask, const C<R>* v, int nvalid_mask, double* racc) {
+  switch (mask) {
+#define BT_CASE(M) \
+  case M: rho_reg<R, M>(v, nvalid_mask, racc); break;
+#define BT_CASE5(M)                                                \
+  case M:                                                          \
+    if constexpr (TileCfg<R>::RB == 5) rho_reg<R, M>(v, nvalid_mask, racc); \
+    break;
+    BT_CASE(1) BT_CASE(2) BT_CASE(4) BT_CASE(8) BT_CASE(3) BT_CASE(5) BT_CASE(6) BT_CASE(9) BT_CASE(10) BT_CASE(12)
+    BT_CASE5(16) BT_CASE5(17) BT_CASE5(18) BT_CASE5(20) BT_CASE5(24)
+#undef BT_CASE
+#undef BT_CASE5
+    default: break;
+  }
+}
+
+// LDS element index swizzle: XOR the low W bits with higher bits so strided
+// layouts spread over the banks (W = 5 for 8-byte, 4 for 16-byte elements).
+template <typename R>
+__host__ __device__ __forceinline__ uint32_t swz(uint32_t l) {
+  constexpr int W = sizeof(C<R>) == 8 ? 5 : 4;
+  return l ^ (((l >> W) ^ (l >> (2 * W))) & ((1u << W) - 1u));
+}
+
+__host__ __device__ __forceinline__ uint32_t ins0(uint32_t x, int b) { return ((x >> b) << (b + 1)) | (x & ((1u << b) - 1u)); }
+
+// k >= 3 operator on the tile in LDS (swizzled), thread tid of the block
+template <typename R, int K>
+__host__ __device__ __forceinline__ void apply_lds(C<R>* lds, int T, const ROp& op, const C<R>* mats, uint32_t tid) {
+  constexpr int NT = TileCfg<R>::NT;
+  constexpr int D = 1 << K;
+  const uint32_t ng = 1u << (T - K);
+  for (uint32_t g = tid; g < ng; g += NT) {
+    uint32_t lb = g;
+#pragma unroll
+    for (int i = 0; i < K; i++) lb = ins0(lb, op.lsorted[i]);
+    C<R> x[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) x[l] = lds[swz<R>(lb + op.loff[l])];
+#pragma unroll 1
+    for (int r = 0; r < D; r++) {  // rows one at a time: the D*D entries stay in LDS
+      C<R> acc = cmake<C<R>>(0, 0);
+#pragma unroll
+      for (int c = 0; c < D; c++) acc = cfma(acc, mats[r + c * D], x[c]);
+      lds[swz<R>(lb + op.loff[r])] = acc;
+    }
+  }
+}
+
+#if defined(__HIPCC__)
+// Block-wide sum of M per-thread values, added to sums[0..M) (LDS) in a
+// fixed order (xor butterfly in each wavefront, then waves in order).
+template <int M, int NT>
+__device__ __forceinline__ void block_add(const double* v, double* red /*LDS [NT/64][M]*/, double* sums) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    double x = v[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) red[wave * M + j] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < M) {
+    double t = 0;
+    for (int w = 0; w < NT / 64; w++) t += red[w * M + threadIdx.x];
+    sums[threadIdx.x] += t;
+  }
+  __syncthreads();
+}
+
+#endif
+
+// Local index of thread t's register r under a layout: thread bits are the
+// local bits not in registers, ascending, holding t's bits in order.
+__host__ __device__ __forceinline__ uint32_t thread_base(const int8_t* rb, int T, uint32_t t) {
+  uint32_t regm = 0;
+#pragma unroll
+  for (int j = 0; j < kRB; j++)
+    if (rb[j] >= 0) regm |= 1u << rb[j];
+  uint32_t base = 0;
+  int i = 0;
+  for (int b = 0; b < T; b++)
+    if (!((regm >> b) & 1)) {
+      base |= ((t >> i) & 1u) << b;
+      i++;
+    }
+  return base;
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void regs_to_lds(const C<R>* v, C<R>* lds, const int8_t* lay, int T, uint32_t tid,
+                                            bool active, int nvalid_mask) {
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  const uint32_t b0 = thread_base(lay, T, tid);
+  uint32_t m[kRB];
+#pragma unroll
+  for (int j = 0; j < kRB; j++) m[j] = lay[j] >= 0 ? 1u << lay[j] : 0u;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    uint32_t l = b0;
+#pragma unroll
+    for (int j = 0; j < kRB; j++)
+      if ((r >> j) & 1) l |= m[j];
+    if (active && !(r & nvalid_mask)) lds[swz<R>(l)] = v[r];
+  }
+}
+
+// Reads are unconditional (straight-line, so they issue back to back):
+// inactive threads pass a mirrored tid < 2^tb and phantom registers alias a
+// valid one; such values are never stored or reduced.
+template <typename R>
+__host__ __device__ __forceinline__ void lds_to_regs(C<R>* v, const C<R>* lds, const int8_t* lay, int T, uint32_t ltid) {
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  const uint32_t b0 = thread_base(lay, T, ltid);
+  uint32_t m[kRB];
+#pragma unroll
+  for (int j = 0; j < kRB; j++) m[j] = lay[j] >= 0 ? 1u << lay[j] : 0u;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    uint32_t l = b0;
+#pragma unroll
+    for (int j = 0; j < kRB; j++)
+      if ((r >> j) & 1) l |= m[j];
+    v[r] = lds[swz<R>(l)];
+  }
+}
+
+// Per-thread constants of a pass: global offsets of the load / store
+// layouts.  Threads >= 2^tb are inactive; their loads mirror thread
+// tid mod 2^tb, so every address is inside the state.
+struct ThreadIO {
+  uint64_t gin[kRB], gout[kRB];
+  uint64_t gthr_in, gthr_out;
+  uint32_t tid, ltid;
+  bool active;
+  int nvalid_mask;  // register bits beyond nrb are phantom
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ void thread_io(const PassGeom& g, uint32_t tid, ThreadIO& io) {
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  io.tid = tid;
+  io.active = tid < (1u << g.tb);
+  io.ltid = tid & ((1u << g.tb) - 1u);
+  io.nvalid_mask = (NR - 1) & ~((1 << g.nrb) - 1);
+#pragma unroll
+  for (int j = 0; j < kRB; j++) {
+    io.gin[j] = g.io_in[j] >= 0 ? 1ull << g.tile[g.io_in[j]] : 0;
+    io.gout[j] = g.io_out[j] >= 0 ? 1ull << g.tile[g.io_out[j]] : 0;
+  }
+  io.gthr_in = 0;
+  io.gthr_out = 0;
+  const uint32_t bi = thread_base(g.io_in, g.T, io.ltid), bo = thread_base(g.io_out, g.T, io.ltid);
+  for (int b = 0; b < g.T; b++) {
+    if ((bi >> b) & 1) io.gthr_in |= 1ull << g.tile[b];
+    if ((bo >> b) & 1) io.gthr_out |= 1ull << g.tile[b];
+  }
+}
+
+// global index of tile t's first amplitude
+__host__ __device__ __forceinline__ uint64_t tile_base(const PassGeom& g, uint64_t t) {
+  uint64_t base = 0;
+  for (int i = 0; i < g.nrun; i++) {
+    const int len = g.run_len[i];
+    base |= (t & ((1ull << len) - 1ull)) << g.run_dst[i];
+    t >>= len;
+  }
+  return base;
+}
+
+// Load a tile's registers (io_in layout), scaled.  The loads are
+// unconditional and straight-line (phantom registers alias a valid
+// address) so they issue back to back; what is not ours is zeroed after.
+// fresh != 0: the state is |0...0> and is not read.
+template <typename R>
+__host__ __device__ __forceinline__ void load_regs(C<R>* v, const C<R>* __restrict__ state, const ThreadIO& io,
+                                                   uint64_t base, R scale, int fresh) {
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  if (fresh) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      uint64_t gi = base | io.gthr_in;
+#pragma unroll
+      for (int j = 0; j < kRB; j++)
+        if ((r >> j) & 1) gi |= io.gin[j];
+      const bool ok = io.active && !(r & io.nvalid_mask);
+      v[r] = cmake<C<R>>(ok && gi == 0 ? scale : (R)0, (R)0);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      uint64_t gi = base | io.gthr_in;
+#pragma unroll
+      for (int j = 0; j < kRB; j++)
+        if ((r >> j) & 1) gi |= io.gin[j];
+      v[r] = state[gi];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const bool ok = io.active && !(r & io.nvalid_mask);
+      v[r].x = ok ? v[r].x * scale : (R)0;
+      v[r].y = ok ? v[r].y * scale : (R)0;
+    }
+  }
+}
+
+// Store a tile's registers (io_out layout) and accumulate FP64 norm^2 into
+// acc[0] and the histogram over g.red_bits into acc[1..16].
+template <typename R>
+__host__ __device__ __forceinline__ void store_regs(const C<R>* v, C<R>* __restrict__ state, const PassGeom& g,
+                                                    const ThreadIO& io, uint64_t base, double* acc) {
+  constexpr int NR = 1 << TileCfg<R>::RB;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    uint64_t gi = base | io.gthr_out;
+#pragma unroll
+    for (int j = 0; j < kRB; j++)
+      if ((r >> j) & 1) gi |= io.gout[j];
+    if (io.active && !(r & io.nvalid_mask)) {
+      if (g.write) state[gi] = v[r];
+      if (g.need_norm) {
+        const double w = (double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y;
+        acc[0] += w;
+        if (g.red == bti::RED_HIST) {
+          int bin = 0;
+          for (int j = 0; j < g.red_k; j++) bin |= (int)((gi >> g.red_bits[j]) & 1ull) << j;
+#pragma unroll
+          for (int b = 0; b < 16; b++) acc[1 + b] += (bin == b) ? w : 0.0;
+        }
+      }
+    }
+  }
+}
+
+__host__ __device__ __forceinline__ int rho_reg_mask(const PassGeom& g) {
+  return (1 << g.red_bits[0]) | (g.red_k > 1 ? 1 << g.red_bits[1] : 0);
+}
+
+#if defined(__HIPCC__)
+// One pass over tiles t = t0, t0 + tstep, ... < tend.  Reductions are added
+// to sums[0..kAcc) (LDS, zeroed by the caller): [0] norm^2, [1..16]
+// histogram, [17..32] packed rho.  lds holds 2^T elements.
+// fresh != 0: the state is |0...0> and is not read (its first pass).
+// The host emulation of this loop (tests/emu) follows it step for step.
+template <typename R>
+__device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGeom& g, const C<R>* mats, C<R>* lds,
+                                          uint64_t t0, uint64_t tstep, uint64_t tend, double* red, double* sums,
+                                          int fresh = 0) {
+  constexpr int NT = TileCfg<R>::NT;
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  const int T = g.T;
+  const R scale = (R)g.scale;
+  ThreadIO io;
+  thread_io<R>(g, threadIdx.x, io);
+  C<R> v[NR];
+  for (uint64_t t = t0; t < tend; t += tstep) {
+    const uint64_t base = tile_base(g, t);
+    load_regs<R>(v, state, io, base, scale, fresh);
+    int8_t cur[kRB];
+#pragma unroll
+    for (int j = 0; j < kRB; j++) cur[j] = g.io_in[j];
+    for (int p = 0; p <= g.nph; p++) {
+      int8_t want[kRB];
+#pragma unroll
+      for (int j = 0; j < kRB; j++) want[j] = p < g.nph ? g.ph[p].rb[j] : g.io_out[j];
+      bool same = true;
+#pragma unroll
+      for (int j = 0; j < kRB; j++) same = same && (cur[j] == want[j]);
+      if (!same) {
+        regs_to_lds<R>(v, lds, cur, T, io.tid, io.active, io.nvalid_mask);
+        __syncthreads();
+        lds_to_regs<R>(v, lds, want, T, io.ltid);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kRB; j++) cur[j] = want[j];
+      }
+      if (p == g.nph) break;
+      const int op0 = g.ph[p].op0, op1 = op0 + g.ph[p].nops;
+      for (int o = op0; o < op1; o++) {
+        const ROp& op = g.ops[o];
+        if ((base & op.gctrl) != op.gctrl) continue;
+        if (op.k <= 2) {
+          apply_mask<R>(op.mask, v, mats + op.moff);
+        } else {
+          regs_to_lds<R>(v, lds, cur, T, io.tid, io.active, io.nvalid_mask);
+          __syncthreads();
+          if (op.k == 3) apply_lds<R, 3>(lds, T, op, mats + op.moff, io.tid);
+          else apply_lds<R, 4>(lds, T, op, mats + op.moff, io.tid);
+          __syncthreads();
+          lds_to_regs<R>(v, lds, cur, T, io.ltid);
+          __syncthreads();
+        }
+      }
+      if (p == g.nph - 1 && g.red == bti::RED_RHO) {
+        double racc[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) racc[j] = 0.0;
+        if (io.active) rho_mask<R>(rho_reg_mask(g), v, io.nvalid_mask, racc);
+        block_add<16, NT>(racc, red, sums + 17);
+      }
+    }
+    double acc[17];
+#pragma unroll
+    for (int j = 0; j < 17; j++) acc[j] = 0.0;
+    store_regs<R>(v, state, g, io, base, acc);
+    if (g.red == bti::RED_HIST) block_add<17, NT>(acc, red, sums);
+    else if (g.need_norm) block_add<1, NT>(acc, red, sums);
+  }
+}
+#endif
+
+// Packed reduced rho (diag, then a < b Re/Im) over k bits -> column-major
+// complex; swap: the packed index's bit 0 is the rho's MSB.
+__host__ __device__ inline void unpack_rho(int k, const double* packed, double* rho, int swap = 0) {
+  const int D = 1 << k;
+  for (int i = 0; i < 2 * D * D; i++) rho[i] = 0.0;
+  auto mp = [&](int a) { return (swap && k == 2) ? (((a & 1) << 1) | (a >> 1)) : a; };
+  for (int a = 0; a < D; a++) rho[2 * (mp(a) + mp(a) * D)] = packed[a];
+  int p = D;
+  for (int a = 0; a < D; a++)
+    for (int b = a + 1; b < D; b++) {
+      const int A = mp(a), B = mp(b);
+      rho[2 * (A + B * D)] = packed[p];
+      rho[2 * (A + B * D) + 1] = packed[p + 1];
+      rho[2 * (B + A * D)] = packed[p];
+      rho[2 * (B + A * D) + 1] = -packed[p + 1];
+      p += 2;
+    }
+}
+
+__host__ __device__ inline void to_result(const Plan& plan, int red_swap, const double* sum, PassResult* res) {
+  res->norm2 = sum[0];
+  for (int j = 0; j < 16; j++) res->hist[j] = sum[1 + j];
+  for (int j = 0; j < 32; j++) res->rho[j] = 0.0;
+  if (plan.red == bti::RED_RHO) unpack_rho(plan.red_k, sum + 17, res->rho, red_swap);
+}
+
+}  // namespace bt

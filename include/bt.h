@@ -156,11 +156,18 @@ int bt_run_circuit(bt_state s, bt_circuit c, uint64_t seed, uint64_t traj, const
 /* The same circuit on a density matrix (to_rho, ops.jl:806-844); circuits
  * with measurements are rejected as the reference does (hilbert.jl:772). */
 int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats);
-/* ntraj trajectories from |0..0> in one persistent launch, uniforms
- * u(seed, traj0 + t, .).  outcomes: ntraj x outcap, nouts: ntraj.
- * final_states (nullable): ntraj x 2^N complex128 interleaved. */
+/* ntraj trajectories from |0..0>, uniforms u(seed, traj0 + t, .), the
+ * shot loop of run (ops.jl:671-676).  outcomes: ntraj x outcap, nouts: ntraj.
+ * final_states (nullable): ntraj x 2^N complex128 interleaved.
+ * Lockstep rounds: every round runs the next pass of all resident
+ * trajectories in one launch; their interpreters run on host threads. */
 int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes, int64_t outcap,
                  int64_t* nouts, double* final_states, bt_run_stats* stats);
+/* The same contract in ONE persistent launch: each workgroup owns a
+ * trajectory, runs its interpreter on the device (lane 0) and takes the next
+ * trajectory from an atomic counter. */
+int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
+                            int64_t outcap, int64_t* nouts, double* final_states, bt_run_stats* stats);
 
 /* Tile width (bits held in LDS per pass) of the circuit executors for a
  * precision; 0 restores the default (14 for BT_C64, 13 for BT_C128).  Set by

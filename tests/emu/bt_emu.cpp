@@ -2,9 +2,15 @@
 //
 // It drives the product's interpreter (bt_interp.hpp) and circuit analysis
 // (bt_circuit.hpp) exactly as the GPU executors do, but runs each planned
-// pass on a complex128 vector on the host.  It checks that every operator of
-// a pass lies inside the pass's tile bits, so planner mistakes fail here
-// before any GPU time is spent.
+// pass on the host.  It checks that every operator of a pass lies inside the
+// pass's tile bits, so planner mistakes fail here before any GPU time is
+// spent.  Two pass executors:
+//  * exec 0: every operator of the plan on a complex128 vector (planner check);
+//  * exec 1 / 2: the register-tile pass of bt_tile.hpp in complex128 /
+//    complex64, thread by thread: the product's geometry (build_geom) and
+//    per-thread load / relayout / operator / store routines, with each
+//    barrier of the device loop becoming a loop over the workgroup.
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +19,7 @@
 
 #include "bt_circuit.hpp"
 #include "bt_interp.hpp"
+#include "bt_tile.hpp"
 
 using cd = std::complex<double>;
 
@@ -48,6 +55,132 @@ static void apply_op(std::vector<cd>& s, int nbits, const bti::QOp& op) {
 
 extern "C" const char* emu_error(void) { return g_msg.c_str(); }
 
+static int g_exec = 0;
+// geometry statistics over all register-tile passes: passes, operators,
+// operators with k = 1, 2, 3, 4 (in-tile controls counted), phases, relayouts
+// per tile, operators applied through LDS
+static int64_t g_gstat[9];
+
+extern "C" void emu_set_exec(int mode) { g_exec = mode; }
+extern "C" void emu_geom_stats(int64_t* out, int reset) {
+  for (int i = 0; i < 9; i++) out[i] = g_gstat[i];
+  if (reset) memset(g_gstat, 0, sizeof g_gstat);
+}
+
+template <typename R>
+static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& plan, const bti::QOp* slots,
+                         bti::PassResult* res, int fresh) {
+  using namespace bt;
+  constexpr int NT = TileCfg<R>::NT;
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  PassGeom g;
+  memset(&g, 0, sizeof g);
+  std::vector<C<R>> mats(kMatEntries);
+  int rc = build_geom<R>(plan, slots, nbits, &g, mats.data());
+  if (rc) {
+    g_msg = "emulator: build_geom failed";
+    return rc;
+  }
+  g_gstat[0]++;
+  g_gstat[1] += g.nops;
+  for (int o = 0; o < g.nops; o++) g_gstat[1 + g.ops[o].k]++;
+  g_gstat[6] += g.nph;
+  const int T = g.T;
+  const R scale = (R)g.scale;
+  const uint64_t ntiles = 1ull << (nbits - T);
+  std::vector<C<R>> lds((size_t)1 << T);
+  std::vector<std::array<C<R>, NR>> v(NT);
+  std::vector<ThreadIO> io(NT);
+  for (int tid = 0; tid < NT; tid++) thread_io<R>(g, tid, io[tid]);
+  double sums[kAcc];
+  for (int j = 0; j < kAcc; j++) sums[j] = 0.0;
+  for (uint64_t t = 0; t < ntiles; t++) {
+    const uint64_t base = tile_base(g, t);
+    for (int tid = 0; tid < NT; tid++) load_regs<R>(v[tid].data(), st.data(), io[tid], base, scale, fresh);
+    int8_t cur[kRB];
+    for (int j = 0; j < kRB; j++) cur[j] = g.io_in[j];
+    for (int p = 0; p <= g.nph; p++) {
+      int8_t want[kRB];
+      for (int j = 0; j < kRB; j++) want[j] = p < g.nph ? g.ph[p].rb[j] : g.io_out[j];
+      bool same = true;
+      for (int j = 0; j < kRB; j++) same = same && (cur[j] == want[j]);
+      if (!same) {
+        if (t == 0) g_gstat[7]++;
+        for (int tid = 0; tid < NT; tid++)
+          regs_to_lds<R>(v[tid].data(), lds.data(), cur, T, tid, io[tid].active, io[tid].nvalid_mask);
+        for (int tid = 0; tid < NT; tid++) lds_to_regs<R>(v[tid].data(), lds.data(), want, T, io[tid].ltid);
+        for (int j = 0; j < kRB; j++) cur[j] = want[j];
+      }
+      if (p == g.nph) break;
+      const int op0 = g.ph[p].op0, op1 = op0 + g.ph[p].nops;
+      for (int o = op0; o < op1; o++) {
+        const ROp& op = g.ops[o];
+        if ((base & op.gctrl) != op.gctrl) continue;
+        if (op.k <= 2) {
+          for (int tid = 0; tid < NT; tid++) apply_mask<R>(op.mask, v[tid].data(), mats.data() + op.moff);
+        } else {
+          if (t == 0) g_gstat[8]++;
+          for (int tid = 0; tid < NT; tid++)
+            regs_to_lds<R>(v[tid].data(), lds.data(), cur, T, tid, io[tid].active, io[tid].nvalid_mask);
+          for (int tid = 0; tid < NT; tid++) {
+            if (op.k == 3) apply_lds<R, 3>(lds.data(), T, op, mats.data() + op.moff, tid);
+            else apply_lds<R, 4>(lds.data(), T, op, mats.data() + op.moff, tid);
+          }
+          for (int tid = 0; tid < NT; tid++) lds_to_regs<R>(v[tid].data(), lds.data(), cur, T, io[tid].ltid);
+        }
+      }
+      if (p == g.nph - 1 && g.red == bti::RED_RHO) {
+        for (int tid = 0; tid < NT; tid++) {
+          double racc[16];
+          for (int j = 0; j < 16; j++) racc[j] = 0.0;
+          if (io[tid].active) rho_mask<R>(rho_reg_mask(g), v[tid].data(), io[tid].nvalid_mask, racc);
+          for (int j = 0; j < 16; j++) sums[17 + j] += racc[j];
+        }
+      }
+    }
+    for (int tid = 0; tid < NT; tid++) {
+      double acc[17];
+      for (int j = 0; j < 17; j++) acc[j] = 0.0;
+      store_regs<R>(v[tid].data(), st.data(), g, io[tid], base, acc);
+      if (g.red == bti::RED_HIST)
+        for (int j = 0; j < 17; j++) sums[j] += acc[j];
+      else if (g.need_norm)
+        sums[0] += acc[0];
+    }
+  }
+  to_result(plan, g.red_swap, sums, res);
+  return BT_OK;
+}
+
+template <typename R>
+static int run_tiles(bti::Interp& ip, const bti::QOp* slots, int nbits, std::vector<std::complex<double>>& s) {
+  const uint64_t len = 1ull << nbits;
+  std::vector<bt::C<R>> st(len);
+  bool zero = true;
+  for (uint64_t i = 0; i < len; i++) {
+    st[i].x = (R)s[i].real();
+    st[i].y = (R)s[i].imag();
+    if (s[i] != (i == 0 ? std::complex<double>(1, 0) : std::complex<double>(0, 0))) zero = false;
+  }
+  int fresh = zero ? 1 : 0;  // as the batched executors: |0..0> is not read on its first pass
+  bti::Plan plan;
+  bti::PassResult res;
+  memset(&res, 0, sizeof res);
+  int guard = 0;
+  while (ip.step(&plan, &res)) {
+    if (++guard > 1000000) {
+      g_msg = "emulator: too many passes";
+      return BT_ERR_STATE;
+    }
+    memset(&res, 0, sizeof res);
+    const int rc = tile_pass_emu<R>(st, nbits, plan, slots, &res, fresh);
+    if (rc) return rc;
+    if (plan.write) fresh = 0;
+  }
+  for (uint64_t i = 0; i < len; i++) s[i] = std::complex<double>(st[i].x, st[i].y);
+  return BT_OK;
+}
+
 // stats: passes, reduce passes, boundary draws, draws used, overflow
 extern "C" int emu_run(int nbits, const bt_ir_op* ops, int nops, int ntop, const double* pool, int64_t npool,
                        uint64_t seed, uint64_t traj, int T, int cmin, int fold, double* state, int* outs,
@@ -71,7 +204,11 @@ extern "C" int emu_run(int nbits, const bt_ir_op* ops, int nops, int ntop, const
   bti::Plan plan;
   bti::PassResult res;
   int guard = 0;
-  while (ip.step(&plan, &res)) {
+  if (g_exec) {
+    rc = g_exec == 1 ? run_tiles<double>(ip, slots.data(), nbits, s) : run_tiles<float>(ip, slots.data(), nbits, s);
+    if (rc) return rc;
+  }
+  while (!g_exec && ip.step(&plan, &res)) {
     if (++guard > 1000000) {
       g_msg = "emulator: too many passes";
       return BT_ERR_STATE;

@@ -58,9 +58,10 @@ def test_run_circuit_random(bt, tiles, prec, T):
         assert np.allclose(s, r, rtol=0, atol=TOL[prec], equal_nan=True), (seed, np.nanmax(np.abs(s - r)))
 
 
+@pytest.mark.parametrize("mode", ["rounds", "persistent"])
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 @pytest.mark.parametrize("T", [0, 5])
-def test_run_batch_matches_per_trajectory_oracle(bt, tiles, prec, T):
+def test_run_batch_matches_per_trajectory_oracle(bt, tiles, prec, T, mode):
     from bluetangle_jl_amd.circuit import Circuit
     tiles(T)
     N = 8
@@ -68,7 +69,7 @@ def test_run_batch_matches_per_trajectory_oracle(bt, tiles, prec, T):
     nm_args = ("depolarizing", 1e-3, "depolarizing", 1e-2)
     c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
     ntraj = 48
-    outs, states, stats = c.run_batch(ntraj, seed=21, prec=prec, keep_states=True)
+    outs, states, stats = c.run_batch(ntraj, seed=21, prec=prec, keep_states=True, mode=mode)
     assert stats["passes"] > 0
     skipped = 0
     for t in range(ntraj):
@@ -81,8 +82,9 @@ def test_run_batch_matches_per_trajectory_oracle(bt, tiles, prec, T):
     assert skipped < ntraj // 4
 
 
-def test_run_batch_random_ops(bt, tiles):
-    """Every op kind through the persistent kernel (c128)."""
+@pytest.mark.parametrize("mode", ["rounds", "persistent"])
+def test_run_batch_random_ops(bt, tiles, mode):
+    """Every op kind through both batched executors (c128)."""
     from bluetangle_jl_amd.circuit import Circuit
     tiles(3)
     rng = np.random.default_rng(7)
@@ -90,7 +92,7 @@ def test_run_batch_random_ops(bt, tiles):
     spec = random_spec(N, 60, rng, kinds=KINDS)
     nm_args = ("depolarizing", 0.05, "amplitude_damping", 0.1)
     c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
-    outs, states, _ = c.run_batch(16, seed=3, prec="c128", keep_states=True)
+    outs, states, _ = c.run_batch(16, seed=3, prec="c128", keep_states=True, mode=mode)
     for t in range(16):
         r, routs, bnd = oracle_traj(N, spec, nm_args, 3, t)
         if bnd:
