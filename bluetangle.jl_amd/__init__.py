@@ -11,10 +11,12 @@ from .noise import (Noise1, Noise2, NoiseModel, QuantumChannel, custom_noise, is
 from .ops import Op, OpF, OpQC, QuantumOps, ifOp
 from .core import (DensityMatrix, DeviceState, StateVector, UniformStream, apply, apply_noise, current_stream,
                    from_numpy, get_N, init, sample_exact, seed, zero_rho, zero_state)
+from .circuit import Circuit, Server, compile_ir, run, to_rho, to_state
 
 __all__ = [
     "BTError", "gate", "gates", "Noise1", "Noise2", "NoiseModel", "QuantumChannel", "custom_noise",
     "is_valid_quantum_channel", "noise_model", "noise_model1", "noise_model2", "Op", "OpF", "OpQC", "QuantumOps",
     "ifOp", "DensityMatrix", "DeviceState", "StateVector", "UniformStream", "apply", "apply_noise",
     "current_stream", "from_numpy", "get_N", "init", "sample_exact", "seed", "zero_rho", "zero_state",
+    "Circuit", "Server", "compile_ir", "run", "to_rho", "to_state",
 ]

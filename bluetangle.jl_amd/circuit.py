@@ -269,3 +269,66 @@ def run(ops_or_circuit, number_of_experiment: int = 1, N=None, noise=False, seed
         o, _ = c.run_on(st, seed, t)
         res.append(o)
     return res
+
+
+class Server:
+    """Continuous-batching executor (bt_server_*): requests (circuit, kind,
+    precision, seed, traj) are admitted while their states fit in the HBM
+    budget and advance together, one launch per round; finished requests
+    leave and queued ones take their place.  A request's result does not
+    depend on what else is resident."""
+
+    def __init__(self, hbm_budget_bytes=0, max_slots=0):
+        init()
+        h = ctypes.c_void_p()
+        L.check(L.lib().bt_server_create(int(hbm_budget_bytes), int(max_slots), ctypes.byref(h)))
+        self.handle = h
+        self._circ = {}  # keep submitted circuits alive until fetched
+
+    def submit(self, circuit: Circuit, kind="sv", prec="c64", seed=0, traj=0) -> int:
+        from .core import PRECISIONS
+        kinds = {"sv": 0, "dm": 1}
+        rid = ctypes.c_int64()
+        L.check(L.lib().bt_server_submit(self.handle, circuit.handle, kinds[kind], PRECISIONS[prec], seed, traj,
+                                         ctypes.byref(rid)))
+        self._circ[rid.value] = circuit
+        return rid.value
+
+    def poll(self, wait=False, cap=256):
+        """Finished requests: list of dicts (id, status, nout, latency_ms, queue_ms)."""
+        res = (L.Result * cap)()
+        n = ctypes.c_int()
+        L.check(L.lib().bt_server_poll(self.handle, res, cap, 1 if wait else 0, ctypes.byref(n)))
+        return [dict(id=r.id, status=r.status, nout=r.nout, latency_ms=r.latency_ms, queue_ms=r.queue_ms)
+                for r in res[: n.value]]
+
+    def fetch(self, rid: int, probs=True, outcap=None):
+        """(outcomes, probabilities or None) of a finished request; frees its state."""
+        c = self._circ.get(rid)
+        N = c.N if c is not None else 0
+        outcap = outcap or (max(16, 2 * c.nops) if c is not None else 4096)
+        outs = np.zeros(outcap, dtype=np.int32)
+        nout = ctypes.c_int64()
+        p = np.zeros(2 ** N, dtype=np.float64) if probs else None
+        L.check(L.lib().bt_server_fetch(self.handle, rid, L.iptr(outs), outcap, ctypes.byref(nout),
+                                        L.dptr(p) if probs else None, p.size if probs else 0))
+        self._circ.pop(rid, None)
+        return outs[: min(outcap, nout.value)].tolist(), p
+
+    def stats(self):
+        a, q = ctypes.c_int(), ctypes.c_int()
+        u = ctypes.c_uint64()
+        L.check(L.lib().bt_server_stats(self.handle, ctypes.byref(a), ctypes.byref(q), ctypes.byref(u)))
+        return dict(active=a.value, queued=q.value, hbm_in_use=u.value)
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h is not None and L._lib is not None:
+            L._lib.bt_server_destroy(h)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

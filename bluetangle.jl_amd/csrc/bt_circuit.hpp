@@ -361,6 +361,7 @@ inline int lower_dm(const CircuitHost& c, CircuitHost* out, std::string* msg) {
 // A validated circuit and its device copies (uploaded on first use).
 struct bt_circuit_s {
   btc::CircuitHost h;
+  uint64_t uid = 0;  // process-unique, never reused (server caches key on it)
   // device copies (uploaded on first use)
   bt_ir_op* d_ops = nullptr;
   double* d_pool = nullptr;

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -361,7 +362,9 @@ int bt_circuit_create(int nqubits, const bt_ir_op* ops, int nops, int ntop, cons
                       bt_circuit* out) {
   if (!out) return fail(BT_ERR_ARG, "null output");
   *out = nullptr;
+  static std::atomic<uint64_t> next_uid{1};
   auto* c = new bt_circuit_s();
+  c->uid = next_uid.fetch_add(1);
   std::string msg;
   int rc = btc::analyze(nqubits, ops, nops, ntop, pool, npool, &c->h, &msg);
   if (rc) {

@@ -575,3 +575,393 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
                                             std::to_string(first_err) + ")");
   return BT_OK;
 }
+
+// ---------------------------------------------------------------------------
+// serving: continuous batching with HBM-budget admission
+// ---------------------------------------------------------------------------
+// A worker thread owns the round engines.  Each loop it admits queued
+// requests in FIFO order while their states fit in the HBM budget (and the
+// slot cap), runs one round over every resident request (one launch per
+// precision), and retires finished ones.  A finished request keeps its state
+// (and its share of the budget) until bt_server_fetch reads it.  Requests
+// never influence each other's results: each has its own uniforms
+// u(seed, traj, .) and its reductions are summed in a fixed order that does
+// not depend on what else is resident.
+namespace bt {
+
+template <typename R>
+__global__ void k_probs(const C<R>* s, uint64_t n, int dm, double* out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (dm) {
+      out[i] = (double)s[i * (n + 1)].x;  // rho[i, i] (column-major, row bits low)
+    } else {
+      const C<R> v = s[i];
+      out[i] = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    }
+  }
+}
+
+struct Request {
+  int64_t id = 0;
+  int kind = BT_SV, prec = BT_C64, nq = 0;
+  uint64_t seed = 0, traj = 0;
+  size_t bytes = 0;
+  std::shared_ptr<const btc::CircuitHost> circ;
+  std::unique_ptr<Job> job;
+  std::vector<int32_t> outs;
+  int64_t nout = 0;
+  int fresh = 1;
+  double t_submit = 0, t_admit = 0;
+  bt_result res{};
+};
+
+}  // namespace bt
+
+struct bt_server_s {
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::thread worker;
+  bool stop = false;
+  int device = 0;
+  uint64_t budget = 0;
+  int max_slots = 0;
+  int64_t next_id = 1;
+  std::deque<std::shared_ptr<bt::Request>> queue;
+  int nrunning = 0;
+  std::deque<bt_result> unpolled;
+  std::unordered_map<int64_t, std::shared_ptr<bt::Request>> finished;
+  uint64_t in_use = 0;   // states of resident + finished-unfetched requests
+  uint64_t cached = 0;   // bytes in the free lists
+  std::unordered_map<size_t, std::vector<void*>> free_bufs;
+  std::vector<void*> to_release;  // freed by the worker (it owns allocation)
+  std::unordered_map<uint64_t, std::shared_ptr<const btc::CircuitHost>> sv_cache, dm_cache;
+  std::unique_ptr<bt::RoundEngine> eng[2];
+  hipStream_t fetch_stream = nullptr;
+  double* d_probs = nullptr;
+  size_t cap_probs = 0;
+  std::mutex fetch_mu;
+};
+
+namespace bt {
+
+static size_t state_bytes(int kind, int prec, int nq) {
+  const int nbits = kind == BT_DM ? 2 * nq : nq;
+  return (size_t)(1ull << nbits) * (prec == BT_C64 ? 8 : 16);
+}
+
+// caller holds srv->mu; only the worker thread calls this
+static void* take_buffer(bt_server_s* S, size_t bytes) {
+  auto it = S->free_bufs.find(bytes);
+  if (it != S->free_bufs.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    S->cached -= bytes;
+    return p;
+  }
+  // release cached buffers of other sizes until the new one fits the budget
+  for (auto& kv : S->free_bufs) {
+    while (!kv.second.empty() && S->in_use + S->cached + bytes > S->budget) {
+      (void)hipFree(kv.second.back());
+      kv.second.pop_back();
+      S->cached -= kv.first;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+
+static void finish_request(bt_server_s* S, Request* r, int err) {
+  Job* j = r->job.get();
+  r->res.id = r->id;
+  r->res.status = err ? -std::abs(err) : 0;
+  if (j && j->ip) {
+    r->nout = j->ip->nout;
+    r->outs.assign(j->outs.begin(), j->outs.begin() + std::min<int64_t>((int64_t)j->outs.size(), r->nout));
+    r->fresh = j->fresh;
+  }
+  r->res.nout = (int32_t)r->nout;
+  const double t = now_ms();
+  r->res.latency_ms = t - r->t_submit;
+  r->res.queue_ms = r->t_admit - r->t_submit;
+  (void)S;
+}
+
+static void server_loop(bt_server_s* S) {
+  (void)hipSetDevice(S->device);
+  std::vector<std::shared_ptr<Request>> running;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      for (void* p : S->to_release) (void)hipFree(p);
+      S->to_release.clear();
+      // admission, FIFO
+      while (!S->queue.empty() && (int)running.size() < S->max_slots) {
+        std::shared_ptr<Request> r = S->queue.front();
+        if (r->bytes > S->budget) {  // can never fit
+          S->queue.pop_front();
+          r->t_admit = now_ms();
+          finish_request(S, r.get(), BT_ERR_NOMEM);
+          S->finished[r->id] = r;
+          S->unpolled.push_back(r->res);
+          S->cv_done.notify_all();
+          continue;
+        }
+        if (S->in_use + r->bytes > S->budget) break;
+        void* mem = take_buffer(S, r->bytes);
+        if (!mem) {
+          if (!running.empty()) break;  // retry when a resident request retires
+          S->queue.pop_front();         // nothing resident: HBM is really short
+          r->t_admit = now_ms();
+          finish_request(S, r.get(), BT_ERR_NOMEM);
+          S->finished[r->id] = r;
+          S->unpolled.push_back(r->res);
+          S->cv_done.notify_all();
+          continue;
+        }
+        S->queue.pop_front();
+        S->in_use += r->bytes;
+        r->t_admit = now_ms();
+        const int e = r->prec == BT_C64 ? 0 : 1;
+        if (!S->eng[e]) {
+          S->eng[e].reset(new RoundEngine(r->prec));
+          if (S->eng[e]->init()) S->eng[e].reset();
+        }
+        r->job.reset(new Job());
+        Job* j = r->job.get();
+        j->id = r->id;
+        j->circ = r->circ;
+        j->nbits = r->kind == BT_DM ? 2 * r->nq : r->nq;
+        j->prec = r->prec;
+        j->fold = r->kind == BT_DM ? 0 : 1;
+        j->seed = r->seed;
+        j->traj = r->traj;
+        j->state = mem;
+        j->bytes = r->bytes;
+        if (!S->eng[e]) {
+          j->done = 1;
+          j->err = BT_ERR_HIP;
+        } else {
+          S->eng[e]->start_job(j);
+        }
+        running.push_back(r);
+      }
+      S->nrunning = (int)running.size();
+      if (running.empty()) {
+        if (S->stop) return;
+        S->cv_work.wait(lk, [&] {
+          return S->stop || (!S->queue.empty() &&
+                             (S->queue.front()->bytes > S->budget || S->in_use + S->queue.front()->bytes <= S->budget));
+        });
+        continue;
+      }
+      if (S->stop) {
+        for (auto& r : running) {
+          S->free_bufs[r->bytes].push_back(r->job->state);
+          S->cached += r->bytes;
+          S->in_use -= r->bytes;
+        }
+        running.clear();
+        S->nrunning = 0;
+        return;
+      }
+    }
+    // one round per precision over the resident requests
+    for (int e = 0; e < 2; e++) {
+      if (!S->eng[e]) continue;
+      std::vector<Job*> jobs;
+      for (auto& r : running)
+        if ((r->prec == BT_C64 ? 0 : 1) == e && !r->job->done) jobs.push_back(r->job.get());
+      if (jobs.empty()) continue;
+      const int rc = S->eng[e]->round(jobs);
+      if (rc)
+        for (Job* j : jobs) {
+          j->done = 1;
+          if (!j->err) j->err = rc;
+        }
+    }
+    // retire
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::vector<std::shared_ptr<Request>> still;
+    for (auto& r : running) {
+      if (!r->job->done) {
+        still.push_back(r);
+        continue;
+      }
+      finish_request(S, r.get(), r->job->err);
+      r->job->ip.reset();
+      S->finished[r->id] = r;
+      S->unpolled.push_back(r->res);
+    }
+    if (still.size() != running.size()) S->cv_done.notify_all();
+    running.swap(still);
+    S->nrunning = (int)running.size();
+  }
+}
+
+}  // namespace bt
+
+extern "C" int bt_server_create(uint64_t hbm_budget_bytes, int max_slots, bt_server* out) {
+  if (!out) return fail(BT_ERR_ARG, "null output");
+  *out = nullptr;
+  int rc = ensure_device();
+  if (rc) return rc;
+  size_t freeb = 0, totalb = 0;
+  BT_HIP(hipMemGetInfo(&freeb, &totalb));
+  auto* S = new bt_server_s();
+  S->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(freeb / 4 * 3);
+  S->max_slots = max_slots > 0 ? max_slots : 1 << 20;
+  (void)hipGetDevice(&S->device);
+  if (hipStreamCreate(&S->fetch_stream) != hipSuccess) {
+    delete S;
+    return fail(BT_ERR_HIP, "bt_server_create: stream");
+  }
+  S->worker = std::thread(server_loop, S);
+  *out = S;
+  return BT_OK;
+}
+
+extern "C" int bt_server_submit(bt_server S, bt_circuit c, int kind, int prec, uint64_t seed, uint64_t traj,
+                                int64_t* id) {
+  if (!S || !c || !id) return fail(BT_ERR_ARG, "null server, circuit or id");
+  if (kind != BT_SV && kind != BT_DM) return fail(BT_ERR_ARG, "bad state kind");
+  if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
+  auto r = std::make_shared<Request>();
+  r->kind = kind;
+  r->prec = prec;
+  r->nq = c->h.n;
+  r->seed = seed;
+  r->traj = traj;
+  r->bytes = state_bytes(kind, prec, c->h.n);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto& cache = kind == BT_DM ? S->dm_cache : S->sv_cache;
+    auto it = cache.find(c->uid);
+    if (it != cache.end()) {
+      r->circ = it->second;
+    } else {
+      auto h = std::make_shared<btc::CircuitHost>();
+      if (kind == BT_DM) {
+        std::string msg;
+        const int rc = lower_dm(c->h, h.get(), &msg);
+        if (rc) return fail(rc, msg);
+      } else {
+        *h = c->h;
+      }
+      cache[c->uid] = h;
+      r->circ = h;
+    }
+    r->id = S->next_id++;
+    r->t_submit = now_ms();
+    *id = r->id;
+    S->queue.push_back(r);
+  }
+  S->cv_work.notify_all();
+  return BT_OK;
+}
+
+extern "C" int bt_server_poll(bt_server S, bt_result* results, int cap, int wait, int* n) {
+  if (!S || !n || (cap > 0 && !results)) return fail(BT_ERR_ARG, "bad poll arguments");
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (wait)
+    S->cv_done.wait(lk, [&] { return !S->unpolled.empty() || (S->queue.empty() && S->nrunning == 0); });
+  int k = 0;
+  while (k < cap && !S->unpolled.empty()) {
+    results[k++] = S->unpolled.front();
+    S->unpolled.pop_front();
+  }
+  *n = k;
+  return BT_OK;
+}
+
+extern "C" int bt_server_fetch(bt_server S, int64_t id, int* outcomes, int64_t outcap, int64_t* nout, double* probs,
+                               uint64_t nprobs) {
+  if (!S) return fail(BT_ERR_ARG, "null server");
+  std::shared_ptr<Request> r;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->finished.find(id);
+    if (it == S->finished.end()) return fail(BT_ERR_ARG, "unknown or unfinished request");
+    r = it->second;
+  }
+  const uint64_t n = 1ull << r->nq;
+  if (probs && nprobs < n) return fail(BT_ERR_ARG, "probability buffer too small");
+  if (nout) *nout = r->nout;
+  if (outcomes)
+    for (int64_t i = 0; i < std::min<int64_t>(outcap, (int64_t)r->outs.size()); i++) outcomes[i] = r->outs[i];
+  int rc = BT_OK;
+  if (probs && r->job && r->res.status == 0) {
+    std::lock_guard<std::mutex> fl(S->fetch_mu);
+    if (r->fresh) {  // never written: |0..0>
+      for (uint64_t i = 0; i < n; i++) probs[i] = i == 0 ? 1.0 : 0.0;
+    } else {
+      if (n > S->cap_probs) {
+        if (S->d_probs) (void)hipFree(S->d_probs);
+        S->d_probs = nullptr;
+        if (hipMalloc((void**)&S->d_probs, sizeof(double) * n) != hipSuccess) {
+          S->cap_probs = 0;
+          return fail(BT_ERR_NOMEM, "bt_server_fetch: probability buffer");
+        }
+        S->cap_probs = n;
+      }
+      const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 16);
+      const int dm = r->kind == BT_DM ? 1 : 0;
+      if (r->prec == BT_C64)
+        hipLaunchKernelGGL(k_probs<float>, dim3(grid), dim3(256), 0, S->fetch_stream, (const C<float>*)r->job->state,
+                           n, dm, S->d_probs);
+      else
+        hipLaunchKernelGGL(k_probs<double>, dim3(grid), dim3(256), 0, S->fetch_stream,
+                           (const C<double>*)r->job->state, n, dm, S->d_probs);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(probs, S->d_probs, sizeof(double) * n, hipMemcpyDeviceToHost, S->fetch_stream) !=
+              hipSuccess ||
+          hipStreamSynchronize(S->fetch_stream) != hipSuccess)
+        rc = fail(BT_ERR_HIP, "bt_server_fetch: probabilities");
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (r->job && r->job->state) {
+      S->free_bufs[r->bytes].push_back(r->job->state);
+      S->cached += r->bytes;
+      S->in_use -= r->bytes;
+      r->job->state = nullptr;
+    }
+    S->finished.erase(id);
+  }
+  S->cv_work.notify_all();
+  return rc;
+}
+
+extern "C" int bt_server_stats(bt_server S, int* active, int* queued, uint64_t* hbm_in_use) {
+  if (!S) return fail(BT_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (active) *active = S->nrunning;
+  if (queued) *queued = (int)S->queue.size();
+  if (hbm_in_use) *hbm_in_use = S->in_use;
+  return BT_OK;
+}
+
+extern "C" int bt_server_destroy(bt_server S) {
+  if (!S) return BT_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  for (auto& kv : S->finished)
+    if (kv.second->job && kv.second->job->state) (void)hipFree(kv.second->job->state);
+  for (auto& kv : S->free_bufs)
+    for (void* p : kv.second) (void)hipFree(p);
+  for (void* p : S->to_release) (void)hipFree(p);
+  for (int e = 0; e < 2; e++) S->eng[e].reset();
+  if (S->d_probs) (void)hipFree(S->d_probs);
+  if (S->fetch_stream) (void)hipStreamDestroy(S->fetch_stream);
+  delete S;
+  return BT_OK;
+}

@@ -1,0 +1,115 @@
+"""GPU tests of the serving executor (bt_server_*): continuous batching with
+HBM-budget admission.  A request's outcomes and probabilities are the same
+bit for bit whether it runs alone or in a mix, agree with the FP64 oracle,
+and the budget bounds what is resident."""
+import numpy as np
+import pytest
+
+import oracle as O
+from refcirc import brickwork_spec, random_spec, to_bt, to_ref
+
+pytestmark = pytest.mark.gpu
+
+NM = ("depolarizing", 0.02, "amplitude_damping", 0.05)
+KINDS = ("gate1", "gate2", "ccx", "chan1", "chan2", "meas", "reset", "ifop")
+
+
+def _circuits(bt):
+    from bluetangle_jl_amd.circuit import Circuit
+    out = []
+    for i, N in enumerate((5, 8, 11)):
+        spec = random_spec(N, 40, np.random.default_rng(900 + i), kinds=KINDS)
+        out.append((spec, N, Circuit(to_bt(spec, bt), N, bt.NoiseModel(*NM))))
+    spec = brickwork_spec(6, 4, 9, ad_gamma=0.02)
+    out.append((spec, 6, Circuit(to_bt(spec, bt), 6, bt.NoiseModel(*NM))))  # also run as a density matrix
+    return out
+
+
+def _drain(srv, n):
+    done = {}
+    while len(done) < n:
+        for r in srv.poll(wait=True):
+            done[r["id"]] = r
+    return done
+
+
+def test_mix_equals_alone_and_oracle(bt):
+    from bluetangle_jl_amd.circuit import Server
+    circs = _circuits(bt)
+    reqs = []  # (circuit index, kind, prec, seed, traj)
+    for t in range(6):
+        for ci in range(3):
+            reqs.append((ci, "sv", "c64" if t % 2 == 0 else "c128", 5, t))
+    reqs.append((3, "dm", "c128", 0, 0))
+    reqs.append((3, "dm", "c64", 0, 0))
+    # the mix
+    srv = Server()
+    ids = [srv.submit(circs[ci][2], kind, prec, seed, traj) for ci, kind, prec, seed, traj in reqs]
+    res = _drain(srv, len(ids))
+    mixed = {}
+    for rid, q in zip(ids, reqs):
+        assert res[rid]["status"] == 0, res[rid]
+        assert 0 <= res[rid]["queue_ms"] <= res[rid]["latency_ms"]
+        mixed[rid] = srv.fetch(rid)
+    assert srv.stats() == dict(active=0, queued=0, hbm_in_use=0)
+    srv.close()
+    # each alone
+    for rid, (ci, kind, prec, seed, traj) in zip(ids, reqs):
+        s1 = Server()
+        r1 = s1.submit(circs[ci][2], kind, prec, seed, traj)
+        _drain(s1, 1)
+        outs, p = s1.fetch(r1)
+        s1.close()
+        assert outs == mixed[rid][0]
+        assert np.array_equal(p, mixed[rid][1]), (ci, kind, prec, traj)
+    # against the oracle
+    for rid, (ci, kind, prec, seed, traj) in zip(ids, reqs):
+        spec, N, _ = circs[ci]
+        outs, p = mixed[rid]
+        tol = 1e-12 if prec == "c128" else 1e-5
+        if kind == "dm":
+            ref = O.run_dm(N, to_ref(spec), O.ref_noise(*NM))
+            assert np.abs(p - np.diag(ref).real).max() < tol
+            continue
+        u = O.uniforms(seed, traj, 16 * len(spec) + 16)
+        r, routs, used, bnd, _ = O.run_sv(N, to_ref(spec), O.ref_noise(*NM), u, tol=1e-6)
+        if bnd:
+            continue
+        assert outs == routs
+        assert np.allclose(p, np.abs(r) ** 2, rtol=0, atol=tol, equal_nan=True)
+
+
+def test_budget_admission_and_rejection(bt):
+    from bluetangle_jl_amd.circuit import Circuit, Server
+    N = 16
+    spec = brickwork_spec(N, 6, 3, mz_prob=0.2)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel("depolarizing", 1e-3, "depolarizing", 1e-2))
+    state = (1 << N) * 8
+    srv = Server(hbm_budget_bytes=2 * state)  # two complex64 states resident at most
+    big = srv.submit(c, "sv", "c128", 1, 0)    # 2x the budget of one... fits alone (2 * 8 B * 2^N)
+    ids = [srv.submit(c, "sv", "c64", 1, t) for t in range(8)]
+    st = srv.stats()
+    assert st["hbm_in_use"] <= 2 * state
+    too_big = Server(hbm_budget_bytes=state // 2)
+    rej = too_big.submit(c, "sv", "c64", 1, 0)
+    r = too_big.poll(wait=True)
+    assert r[0]["id"] == rej and r[0]["status"] < 0
+    too_big.close()
+    done = {}
+    peak = 0
+    while len(done) < len(ids) + 1:
+        peak = max(peak, srv.stats()["hbm_in_use"])
+        for x in srv.poll(wait=True):
+            done[x["id"]] = x
+            outs, _ = srv.fetch(x["id"], probs=False)
+    assert peak <= 2 * state
+    assert all(x["status"] == 0 for x in done.values())
+    # the same trajectories through bt_run_batch give the same outcomes
+    outs_b, _, _ = c.run_batch(8, seed=1, prec="c64")
+    srv2 = Server()
+    ids2 = [srv2.submit(c, "sv", "c64", 1, t) for t in range(8)]
+    _drain(srv2, 8)
+    for t, rid in enumerate(ids2):
+        assert srv2.fetch(rid, probs=False)[0] == outs_b[t]
+    srv2.close()
+    srv.close()
