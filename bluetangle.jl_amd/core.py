@@ -112,6 +112,11 @@ class DeviceState:
         L.check(L.lib().bt_state_set_basis(self.handle, int(index)))
         return self
 
+    def sync(self):
+        """Wait for the work queued on this state's stream."""
+        L.check(L.lib().bt_state_sync(self.handle))
+        return self
+
     def upload(self, arr):
         a = np.ascontiguousarray(np.asarray(arr, dtype=np.complex128).reshape(-1, order="F"))
         if a.size != self.length:

@@ -208,6 +208,13 @@ int bt_state_device_ptr(bt_state s, void** ptr, void** stream) {
   return BT_OK;
 }
 
+int bt_state_sync(bt_state s) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  BT_HIP(hipStreamSynchronize(s->stream));
+  return BT_OK;
+}
+
 int bt_state_set_basis(bt_state s, uint64_t index) {
   int rc = check_state(s);
   if (rc) return rc;

@@ -151,25 +151,21 @@ __global__ void __launch_bounds__(TileCfg<R>::NT) k_grid_pass(C<R>* state, const
   __shared__ C<R> mats[kMatEntries];
   __shared__ double red[(NT / 64) * 17];
   __shared__ double sums[kAcc];
+  load_geom(&tab->g, &geom, threadIdx.x, NT);
   {
-    const int* src = (const int*)&tab->g;
-    int* dst = (int*)&geom;
-    for (int i = threadIdx.x; i < (int)(sizeof(PassGeom) / sizeof(int)); i += NT) dst[i] = src[i];
     const int nm = tab->g.nmats;
     for (int i = threadIdx.x; i < nm; i += NT) mats[i] = tab->mats[i];
   }
   if (threadIdx.x < kAcc) sums[threadIdx.x] = 0.0;
   __syncthreads();
   tile_pass<R>(state, geom, mats, tile, blockIdx.x, gridDim.x, ntiles, red, sums);
-  if (threadIdx.x < kAcc) part[(uint64_t)blockIdx.x * kAcc + threadIdx.x] = sums[threadIdx.x];
+  if (threadIdx.x < kAcc) part[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = sums[threadIdx.x];
 }
 
-__global__ void k_finalize_acc(const double* part, int nblocks, double* out) {
-  for (int j = threadIdx.x; j < kAcc; j += blockDim.x) {
-    double t = 0.0;
-    for (int b = 0; b < nblocks; b++) t += part[(size_t)b * kAcc + j];
-    out[j] = t;
-  }
+// out[j] = fixed-order sum of the grid's partials of accumulator j
+__global__ void __launch_bounds__(256) k_finalize_acc(const double* part, int nblocks, double* out) {
+  const double t = sum_blocks(part + (uint64_t)blockIdx.x * nblocks, (uint32_t)nblocks);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
 }  // namespace bt
@@ -313,7 +309,8 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
       break;
     }
     if (plan.aligned || plan.red != bti::RED_NONE) {
-      hipLaunchKernelGGL(k_finalize_acc, dim3(1), dim3(64), 0, s->stream, (const double*)d_part, (int)blocks, d_acc);
+      hipLaunchKernelGGL(k_finalize_acc, dim3(kAcc), dim3(256), 0, s->stream, (const double*)d_part, (int)blocks,
+                         d_acc);
       if (hipMemcpyAsync(h_acc, d_acc, sizeof(double) * kAcc, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
           hipStreamSynchronize(s->stream) != hipSuccess) {
         rc = fail(BT_ERR_HIP, "run_grid: reduction copy");
@@ -336,6 +333,7 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
     stats->bytes = ip->bytes_units * (double)(1ull << nbits) * (double)s->elem();
     stats->boundary_draws = ip->boundary;
     stats->draws = ip->draw;
+    stats->launches = ip->passes + ip->rpasses;
     float ms = 0;
     if (rc == BT_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) stats->kernel_ms = ms;
   }
@@ -533,6 +531,7 @@ int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0
     stats->reduce_passes = (int64_t)hst[1];
     stats->boundary_draws = (int64_t)hst[2];
     stats->draws = (int64_t)hst[3];
+    stats->launches = 1;
     // bytes: read+write per pass, read per reduction pass, one write to zero
     // each trajectory's state
     stats->bytes = ((double)hst[0] * 2.0 + (double)hst[1] + (double)ntraj) * (double)len * (double)elem;

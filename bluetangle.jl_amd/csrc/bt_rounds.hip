@@ -48,9 +48,13 @@ struct RoundItem {
 
 // table layout: PassGeom, then int32 nmats (padded to 16 B), then nmats matrix entries
 constexpr size_t kGeomBytes = (sizeof(PassGeom) + 15) / 16 * 16;
+constexpr uint32_t kTilesPerBlock = 8;
 
+// One workgroup per `tpb` consecutive tiles of one item.  nb_uniform != 0:
+// every item has that many workgroups (the block -> item map is a division),
+// else a binary search over the items' first blocks.
 template <typename R>
-__global__ void __launch_bounds__(TileCfg<R>::NT) k_round(const RoundItem* items, int nitems,
+__global__ void __launch_bounds__(TileCfg<R>::NT) k_round(const RoundItem* items, int nitems, uint32_t nb_uniform,
                                                           const unsigned char* blob, int tpb, double* part) {
   constexpr int NT = TileCfg<R>::NT;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -60,22 +64,26 @@ __global__ void __launch_bounds__(TileCfg<R>::NT) k_round(const RoundItem* items
   __shared__ double red[(NT / 64) * 17];
   __shared__ double sums[kAcc];
   __shared__ int item_sh;
-  if (threadIdx.x == 0) {
-    int lo = 0, hi = nitems - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (items[mid].block0 <= blockIdx.x) lo = mid;
-      else hi = mid - 1;
+  int item;
+  if (nb_uniform) {
+    item = (int)(blockIdx.x / nb_uniform);
+  } else {
+    if (threadIdx.x == 0) {
+      int lo = 0, hi = nitems - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].block0 <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+      }
+      item_sh = lo;
     }
-    item_sh = lo;
+    __syncthreads();
+    item = item_sh;
   }
-  __syncthreads();
-  const RoundItem it = items[item_sh];
+  const RoundItem it = items[item];
   const unsigned char* tab = blob + it.table;
+  load_geom((const PassGeom*)tab, &geom, threadIdx.x, NT);
   {
-    const int* src = (const int*)tab;
-    int* dst = (int*)&geom;
-    for (int i = threadIdx.x; i < (int)(sizeof(PassGeom) / sizeof(int)); i += NT) dst[i] = src[i];
     const int nm = *(const int*)(tab + kGeomBytes);
     const C<R>* m = (const C<R>*)(tab + kGeomBytes + 16);
     for (int i = threadIdx.x; i < nm; i += NT) mats[i] = m[i];
@@ -86,16 +94,15 @@ __global__ void __launch_bounds__(TileCfg<R>::NT) k_round(const RoundItem* items
   const uint64_t t0 = (uint64_t)local * tpb;
   const uint64_t tend = std::min<uint64_t>((uint64_t)it.ntiles, t0 + tpb);
   tile_pass<R>((C<R>*)it.state, geom, mats, tile, t0, 1, tend, red, sums, it.fresh);
-  if (threadIdx.x < kAcc) part[(uint64_t)blockIdx.x * kAcc + threadIdx.x] = sums[threadIdx.x];
+  if (threadIdx.x < kAcc) part[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = sums[threadIdx.x];
 }
 
-__global__ void k_round_finalize(const RoundItem* items, const double* part, double* out) {
+// out[item, j]: fixed-order sum of the item's partials (grid: items x kAcc)
+__global__ void __launch_bounds__(256) k_round_finalize(const RoundItem* items, const double* part, uint64_t stride,
+                                                        double* out) {
   const RoundItem it = items[blockIdx.x];
-  for (int j = threadIdx.x; j < kAcc; j += blockDim.x) {
-    double t = 0.0;
-    for (uint32_t b = 0; b < it.nblocks; b++) t += part[(uint64_t)(it.block0 + b) * kAcc + j];
-    out[(uint64_t)blockIdx.x * kAcc + j] = t;
-  }
+  const double t = sum_blocks(part + (uint64_t)blockIdx.y * stride + it.block0, it.nblocks);
+  if (threadIdx.x == 0) out[(uint64_t)blockIdx.x * kAcc + blockIdx.y] = t;
 }
 
 template <typename R>
@@ -321,11 +328,16 @@ class RoundEngine {
     for (Job* j : jobs)
       if (j->has_plan) act.push_back(j);
     if (act.empty()) return BT_OK;
-    const int tpb = 2;
+    // tiles per workgroup: enough to amortise the workgroup's set-up, few
+    // enough that the largest item still spreads over the chip
+    uint32_t max_tiles = 0;
+    for (Job* j : act) max_tiles = std::max<uint32_t>(max_tiles, (uint32_t)(1ull << (j->nbits - j->plan.ntile)));
+    const int tpb = (int)std::max<uint32_t>(1, std::min<uint32_t>(kTilesPerBlock, max_tiles / 64));
     size_t blob = 0;
     uint32_t blocks = 0;
     std::vector<RoundItem> items(act.size());
     size_t maxT = 0;
+    uint32_t nb_uniform = 0;
     for (size_t i = 0; i < act.size(); i++) {
       Job* j = act[i];
       const int T = j->plan.ntile;
@@ -337,6 +349,8 @@ class RoundEngine {
       it.block0 = blocks;
       it.nblocks = (it.ntiles + tpb - 1) / tpb;
       it.fresh = j->fresh;
+      if (i == 0) nb_uniform = it.nblocks;
+      else if (nb_uniform != it.nblocks) nb_uniform = 0;
       blocks += it.nblocks;
       blob += (j->table.size() + 15) / 16 * 16;
     }
@@ -354,16 +368,16 @@ class RoundEngine {
     if (prec_ == BT_C64) {
       (void)hipFuncSetAttribute((const void*)k_round<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL((k_round<float>), dim3(blocks), dim3(TileCfg<float>::NT), lds, stream_, d_items_,
-                         (int)items.size(), (const unsigned char*)d_blob_, tpb, d_part_);
+                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, tpb, d_part_);
     } else {
       (void)hipFuncSetAttribute((const void*)k_round<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL((k_round<double>), dim3(blocks), dim3(TileCfg<double>::NT), lds, stream_, d_items_,
-                         (int)items.size(), (const unsigned char*)d_blob_, tpb, d_part_);
+                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, tpb, d_part_);
     }
     BT_HIP(hipGetLastError());
     BT_HIP(hipEventRecord(ev_b_, stream_));
-    hipLaunchKernelGGL(k_round_finalize, dim3((unsigned)items.size()), dim3(64), 0, stream_, d_items_, d_part_,
-                       d_out_);
+    hipLaunchKernelGGL(k_round_finalize, dim3((unsigned)items.size(), kAcc), dim3(256), 0, stream_, d_items_, d_part_,
+                       (uint64_t)blocks, d_out_);
     BT_HIP(hipGetLastError());
     BT_HIP(hipMemcpyAsync(h_out_, d_out_, sizeof(double) * kAcc * items.size(), hipMemcpyDeviceToHost, stream_));
     BT_HIP(hipStreamSynchronize(stream_));
@@ -566,6 +580,7 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
   (void)hipStreamSynchronize(eng.stream());
   // device time of the pass kernels (HIP events around each launch)
   acc.kernel_ms = eng.pass_ms();
+  acc.launches = eng.rounds();
   if (stats) *stats = acc;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "bt_internal.hpp"
 #include "bt_interp.hpp"
 
@@ -649,6 +651,34 @@ __host__ __device__ __forceinline__ int rho_reg_mask(const PassGeom& g) {
 }
 
 #if defined(__HIPCC__)
+// Copy a pass geometry into LDS: the header, the used phases and operators.
+__device__ __forceinline__ void load_geom(const PassGeom* __restrict__ src, PassGeom* dst, int tid, int nt) {
+  constexpr int hdr = (int)(offsetof(PassGeom, ph) / 4);
+  const int nph = src->nph, nops = src->nops;
+  const int* s = (const int*)src;
+  int* d = (int*)dst;
+  const int ph_end = hdr + nph * (int)(sizeof(RPhase) / 4);
+  constexpr int ops0 = (int)(offsetof(PassGeom, ops) / 4);
+  const int ops_end = ops0 + nops * (int)(sizeof(ROp) / 4);
+  for (int i = tid; i < ph_end; i += nt) d[i] = s[i];
+  for (int i = ops0 + tid; i < ops_end; i += nt) d[i] = s[i];
+}
+
+// Fixed-order sum of per-workgroup partials laid out accumulator-major
+// (part[j * stride + block]): one workgroup of 256 threads per (item j).
+__device__ __forceinline__ double sum_blocks(const double* __restrict__ p, uint32_t n) {
+  __shared__ double sh[256];
+  double t = 0.0;
+  for (uint32_t b = threadIdx.x; b < n; b += 256) t += p[b];
+  sh[threadIdx.x] = t;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
 // One pass over tiles t = t0, t0 + tstep, ... < tend.  Reductions are added
 // to sums[0..kAcc) (LDS, zeroed by the caller): [0] norm^2, [1..16]
 // histogram, [17..32] packed rho.  lds holds 2^T elements.

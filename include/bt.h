@@ -76,6 +76,8 @@ int bt_state_upload(bt_state s, const double* data, uint64_t n);
 int bt_state_download(bt_state s, double* data, uint64_t n);
 /* device pointer and HIP stream of a state, for zero-copy interop */
 int bt_state_device_ptr(bt_state s, void** ptr, void** stream);
+/* Wait for the work queued on the state's stream. */
+int bt_state_sync(bt_state s);
 /* Device memory (bytes) a state of this shape would hold. */
 uint64_t bt_state_bytes(int nqubits, int kind, int prec);
 
@@ -144,7 +146,8 @@ typedef struct {
   double bytes;             /* HBM bytes of the passes actually executed */
   int64_t boundary_draws;   /* draws within 1e-6 (c64) / 1e-12 (c128) of a boundary */
   int64_t draws;            /* uniforms consumed */
-  double kernel_ms;         /* device time of the run (HIP events) */
+  double kernel_ms;         /* device time of the pass kernels (HIP events) */
+  int64_t launches;         /* pass-kernel launches (bytes / launches per launch) */
 } bt_run_stats;
 
 /* One trajectory of a state-vector circuit on state s (run / to_state,
