@@ -308,8 +308,9 @@ def _apply_one(state, op, noise, track, pars, rng, outs):
             for o in d:
                 _apply_one(state, o, False, False, pars, rng, None)
             return state
-        v = np.asarray(d) @ state.to_numpy()
-        return state.upload(v)
+        m = _colmajor(np.asarray(d, dtype=np.complex128))
+        L.check(L.lib().bt_apply_full(state.handle, L.dptr(m)))
+        return state
     if isinstance(op, OpQC):
         if dm:
             _channel_dm(state, op.kraus, op.qubit, op.target_qubit)

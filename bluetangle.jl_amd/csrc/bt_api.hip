@@ -120,6 +120,26 @@ static int check_state(bt_state_s* s) {
 
 using namespace bt;
 
+namespace bt {
+// y = M x, one thread per row; M column-major complex128 (a wavefront reads a
+// contiguous piece of each column), FP64 accumulation
+template <typename R>
+__global__ void k_matvec(const double2* __restrict__ M, const C<R>* __restrict__ x, C<R>* __restrict__ y,
+                         uint64_t D) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= D) return;
+  double ar = 0.0, ai = 0.0;
+  for (uint64_t c = 0; c < D; c++) {
+    const double2 m = M[r + c * D];
+    const C<R> v = x[c];
+    ar += m.x * (double)v.x - m.y * (double)v.y;
+    ai += m.x * (double)v.y + m.y * (double)v.x;
+  }
+  y[r].x = (R)ar;
+  y[r].y = (R)ai;
+}
+}  // namespace bt
+
 extern "C" {
 
 const char* bt_last_error(void) { return g_err.c_str(); }
@@ -313,6 +333,41 @@ int bt_apply_matrix(bt_state s, int k, const double* mat, const int* bits, uint6
   rc = launch_apply(s, s->nbits, k, mat, pr, 1.0, false);
   if (rc) return rc;
   return launch_apply(s, s->nbits, k, mat, pc, 1.0, true);
+}
+
+
+int bt_apply_full(bt_state s, const double* mat) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!mat) return fail(BT_ERR_ARG, "null matrix");
+  if (s->kind != BT_SV) return fail(BT_ERR_UNSUPPORTED, "bt_apply_full: state-vector only");
+  if (s->n > 15) return fail(BT_ERR_ARG, "bt_apply_full: a full-space matrix needs N <= 15");
+  const uint64_t D = s->len;
+  void* dM = nullptr;
+  void* dy = nullptr;
+  if (hipMalloc(&dM, sizeof(double2) * D * D) != hipSuccess || hipMalloc(&dy, s->elem() * D) != hipSuccess) {
+    if (dM) (void)hipFree(dM);
+    return fail(BT_ERR_NOMEM, "bt_apply_full: device buffers");
+  }
+  auto done = [&](int r) {
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(dM);
+    (void)hipFree(dy);
+    return r;
+  };
+  if (hipMemcpyAsync(dM, mat, sizeof(double2) * D * D, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    return done(fail(BT_ERR_HIP, "bt_apply_full: matrix upload"));
+  const unsigned grid = (unsigned)((D + 255) / 256);
+  if (s->prec == BT_C64)
+    hipLaunchKernelGGL(k_matvec<float>, dim3(grid), dim3(256), 0, s->stream, (const double2*)dM,
+                       (const C<float>*)s->d, (C<float>*)dy, D);
+  else
+    hipLaunchKernelGGL(k_matvec<double>, dim3(grid), dim3(256), 0, s->stream, (const double2*)dM,
+                       (const C<double>*)s->d, (C<double>*)dy, D);
+  if (hipGetLastError() != hipSuccess) return done(fail(BT_ERR_HIP, "bt_apply_full: launch"));
+  if (hipMemcpyAsync(s->d, dy, s->elem() * D, hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
+    return done(fail(BT_ERR_HIP, "bt_apply_full: copy back"));
+  return done(BT_OK);
 }
 
 int bt_reduced_rho(bt_state s, int k, const int* bits, double* rho_out, double* norm2_out) {

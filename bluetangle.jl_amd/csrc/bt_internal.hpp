@@ -130,7 +130,7 @@ namespace bt {
 constexpr int kRedBlocks = 1024;  // reduction partial blocks
 constexpr int kRedMax = 64 + 2;   // max reals per reduction (8x8 Hermitian + extras)
 
-// psi <- scale * M psi on a placement; M is row-major? no: column-major complex128.
+// psi <- scale * M psi on a placement; M column-major complex128 (conj_mat: conj(M)).
 int launch_apply(bt_state_s* s, int nbits, int k, const double* mat_colmajor, const Placement& p, double scale,
                  bool conj_mat);
 // FP64 reduced density matrix over k bits (k <= 3) into s->h_out (d*d reals:

@@ -86,6 +86,10 @@ uint64_t bt_state_bytes(int nqubits, int kind, int prec);
  * SV: psi <- M psi on bits (k <= 5) where all bits of ctrl_mask are 1.
  * DM: rho <- E rho E' (hilbert.jl:655-656), k <= 3. */
 int bt_apply_matrix(bt_state s, int k, const double* mat, const int* bits, uint64_t ctrl_mask);
+/* OpF(name, mat) (struct.jl:716-720): psi <- M psi with a full 2^N x 2^N
+ * matrix, column-major complex128; FP64 accumulation.  SV states only (the
+ * reference's matrix OpF has no density-matrix method). */
+int bt_apply_full(bt_state s, const double* mat);
 /* Reduced density matrix over bits (k <= 3), FP64, column-major 2^k x 2^k;
  * bits[0] = MSB (partial_trace, linalg.jl:83-230).  SV states only. */
 int bt_reduced_rho(bt_state s, int k, const int* bits, double* rho_out, double* norm2_out);

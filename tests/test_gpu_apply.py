@@ -135,3 +135,26 @@ def test_errors_are_reported(bt):
     rho = bt.zero_rho(2)
     with pytest.raises(ValueError):
         bt.apply(rho, bt.Op("MZ", 1))
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_opf_matrix_and_op_list(bt, prec):
+    """OpF(name, mat) is mat * state (struct.jl:716-720); OpF(name, ops) is the
+    ops applied in order without noise (struct.jl:723-741)."""
+    N = 5
+    rng = np.random.default_rng(17)
+    a = rng.normal(size=(2 ** N, 2 ** N)) + 1j * rng.normal(size=(2 ** N, 2 ** N))
+    q, _ = np.linalg.qr(a)
+    v = rng.normal(size=2 ** N) + 1j * rng.normal(size=2 ** N)
+    v /= np.linalg.norm(v)
+    st = bt.from_numpy(v, prec)
+    bt.apply(st, bt.OpF("U", q))
+    assert np.abs(st.to_numpy() - q @ v).max() < TOL[prec]
+    spec = random_spec(N, 20, rng, kinds=("gate1", "gate2"))
+    ops = to_bt(spec, bt)
+    s1 = bt.from_numpy(v, prec)
+    bt.apply(s1, bt.OpF("seq", ops), noise=bt.NoiseModel("depolarizing", 0.5, "depolarizing", 0.5))
+    r, _, _ = run_oracle(N, spec, None, 0, state=v)
+    assert np.abs(s1.to_numpy() - r).max() < TOL[prec]
+    with pytest.raises(Exception):
+        bt.apply(bt.zero_rho(3, prec), bt.OpF("U", np.eye(8)))
