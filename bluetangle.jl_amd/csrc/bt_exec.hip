@@ -51,13 +51,13 @@ struct BatchArgs {
 };
 
 template <typename R>
-__global__ void __launch_bounds__(TileCfg<R>::NT) k_batch(BatchArgs a) {
+__global__ void __launch_bounds__(TileCfg<R>::NT, TileCfg<R>::WAVES) k_batch(BatchArgs a) {
   constexpr int NT = TileCfg<R>::NT;
   extern __shared__ __align__(16) unsigned char smem[];
   C<R>* tile = (C<R>*)smem;
   __shared__ PassGeom geom;
   __shared__ C<R> mats[kMatEntries];
-  __shared__ double red[(NT / 64) * 17];
+  __shared__ double red[(NT / 64) * kAcc];
   __shared__ double sums[kAcc];
   __shared__ Plan plan;
   __shared__ PassResult res;
@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(TileCfg<R>::NT) k_batch(BatchArgs a) {
       if (!cmd) break;
       if (threadIdx.x < kAcc) sums[threadIdx.x] = 0.0;
       __syncthreads();
-      tile_pass<R>(state, geom, mats, tile, 0, 1, ntiles, red, sums, fresh);
+      tile_pass<R, uint32_t>(state, geom, mats, tile, 0, 1, ntiles, red, sums, fresh);
       if (geom.write) fresh = 0;
       if (threadIdx.x == 0) to_result(plan, geom.red_swap, sums, &res);
       __syncthreads();
@@ -141,15 +141,16 @@ struct GridTable {
   C<R> mats[kMatEntries];
 };
 
-template <typename R>
-__global__ void __launch_bounds__(TileCfg<R>::NT) k_grid_pass(C<R>* state, const GridTable<R>* tab, uint64_t ntiles,
+// O: offset type of the tile pass (uint32_t up to 2^32 amplitudes)
+template <typename R, typename O>
+__global__ void __launch_bounds__(TileCfg<R>::NT, TileCfg<R>::WAVES) k_grid_pass(C<R>* state, const GridTable<R>* tab, uint64_t ntiles,
                                                               double* part) {
   constexpr int NT = TileCfg<R>::NT;
   extern __shared__ __align__(16) unsigned char smem[];
   C<R>* tile = (C<R>*)smem;
   __shared__ PassGeom geom;
   __shared__ C<R> mats[kMatEntries];
-  __shared__ double red[(NT / 64) * 17];
+  __shared__ double red[(NT / 64) * kAcc];
   __shared__ double sums[kAcc];
   load_geom(&tab->g, &geom, threadIdx.x, NT);
   {
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(TileCfg<R>::NT) k_grid_pass(C<R>* state, const
   }
   if (threadIdx.x < kAcc) sums[threadIdx.x] = 0.0;
   __syncthreads();
-  tile_pass<R>(state, geom, mats, tile, blockIdx.x, gridDim.x, ntiles, red, sums);
+  tile_pass<R, O>(state, geom, mats, tile, blockIdx.x, gridDim.x, ntiles, red, sums);
   if (threadIdx.x < kAcc) part[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = sums[threadIdx.x];
 }
 
@@ -202,6 +203,10 @@ int default_tile_bits(int prec) {
   return prec == BT_C64 ? TileCfg<float>::LOGNT + TileCfg<float>::RB : TileCfg<double>::LOGNT + TileCfg<double>::RB;
 }
 
+// Low global bits every tile keeps (runs of 2^c consecutive amplitudes per
+// wavefront access): 128 B for either precision.
+int coalesce_bits(int prec) { return prec == BT_C64 ? 4 : 3; }
+
 int tile_bits_for(int prec, int nbits) {
   int T = g_tile_override[prec] ? g_tile_override[prec] : default_tile_bits(prec);
   return std::min(T, nbits);
@@ -217,7 +222,7 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
   std::unique_ptr<bti::Interp> ip(new bti::Interp());
   bti::Program prog{ch.ops.data(), ch.ntop, ch.pool.data(), ch.chan.data(), ch.chanw.data(), nbits};
   const int T = prec_tile_T;
-  ip->init(prog, slots.data(), T, 3);
+  ip->init(prog, slots.data(), T, coalesce_bits(s->prec));
   ip->seed = seed;
   ip->traj = traj;
   if (uni) {
@@ -253,12 +258,17 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
   if (rc == BT_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(BT_ERR_HIP, "event");
   if (rc == BT_OK) {
-    if (s->prec == BT_C64)
-      (void)hipFuncSetAttribute((const void*)k_grid_pass<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (s->prec == BT_C64) {
+      (void)hipFuncSetAttribute((const void*)k_grid_pass<float, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
-    else
-      (void)hipFuncSetAttribute((const void*)k_grid_pass<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (void)hipFuncSetAttribute((const void*)k_grid_pass<float, uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
+    } else {
+      (void)hipFuncSetAttribute((const void*)k_grid_pass<double, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+      (void)hipFuncSetAttribute((const void*)k_grid_pass<double, uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+    }
     (void)hipEventRecord(e0, s->stream);
   }
   Plan plan;
@@ -298,12 +308,22 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
       break;
     }
     (void)hipEventRecord(ev[slot], s->stream);
-    if (s->prec == BT_C64)
-      hipLaunchKernelGGL((k_grid_pass<float>), dim3(blocks), dim3(TileCfg<float>::NT), lds, s->stream,
-                         (C<float>*)s->d, (const GridTable<float>*)dslot, ntiles, d_part);
-    else
-      hipLaunchKernelGGL((k_grid_pass<double>), dim3(blocks), dim3(TileCfg<double>::NT), lds, s->stream,
-                         (C<double>*)s->d, (const GridTable<double>*)dslot, ntiles, d_part);
+    const bool narrow = nbits <= 32;
+    if (s->prec == BT_C64) {
+      if (narrow)
+        hipLaunchKernelGGL((k_grid_pass<float, uint32_t>), dim3(blocks), dim3(TileCfg<float>::NT), lds, s->stream,
+                           (C<float>*)s->d, (const GridTable<float>*)dslot, ntiles, d_part);
+      else
+        hipLaunchKernelGGL((k_grid_pass<float, uint64_t>), dim3(blocks), dim3(TileCfg<float>::NT), lds, s->stream,
+                           (C<float>*)s->d, (const GridTable<float>*)dslot, ntiles, d_part);
+    } else {
+      if (narrow)
+        hipLaunchKernelGGL((k_grid_pass<double, uint32_t>), dim3(blocks), dim3(TileCfg<double>::NT), lds,
+                           s->stream, (C<double>*)s->d, (const GridTable<double>*)dslot, ntiles, d_part);
+      else
+        hipLaunchKernelGGL((k_grid_pass<double, uint64_t>), dim3(blocks), dim3(TileCfg<double>::NT), lds,
+                           s->stream, (C<double>*)s->d, (const GridTable<double>*)dslot, ntiles, d_part);
+    }
     if (hipGetLastError() != hipSuccess) {
       rc = fail(BT_ERR_HIP, "run_grid: pass launch");
       break;
@@ -487,7 +507,7 @@ int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0
   a.prog = bti::Program{c->d_ops, c->h.ntop, c->d_pool, c->d_chan, c->d_chanw, nbits};
   a.nbits = nbits;
   a.T = T;
-  a.cmin = 3;
+  a.cmin = coalesce_bits(prec);
   a.want_final = final_states ? 1 : 0;
   a.seed = seed;
   a.traj0 = traj0;

@@ -33,6 +33,7 @@
 namespace bt {
 
 int tile_bits_for(int prec, int nbits);
+int coalesce_bits(int prec);
 
 // ---------------------------------------------------------------------------
 // device side
@@ -54,14 +55,14 @@ constexpr uint32_t kTilesPerBlock = 8;
 // every item has that many workgroups (the block -> item map is a division),
 // else a binary search over the items' first blocks.
 template <typename R>
-__global__ void __launch_bounds__(TileCfg<R>::NT) k_round(const RoundItem* items, int nitems, uint32_t nb_uniform,
+__global__ void __launch_bounds__(TileCfg<R>::NT, TileCfg<R>::WAVES) k_round(const RoundItem* items, int nitems, uint32_t nb_uniform,
                                                           const unsigned char* blob, int tpb, double* part) {
   constexpr int NT = TileCfg<R>::NT;
   extern __shared__ __align__(16) unsigned char smem[];
   C<R>* tile = (C<R>*)smem;
   __shared__ PassGeom geom;
   __shared__ C<R> mats[kMatEntries];
-  __shared__ double red[(NT / 64) * 17];
+  __shared__ double red[(NT / 64) * kAcc];
   __shared__ double sums[kAcc];
   __shared__ int item_sh;
   int item;
@@ -93,7 +94,7 @@ __global__ void __launch_bounds__(TileCfg<R>::NT) k_round(const RoundItem* items
   const uint32_t local = blockIdx.x - it.block0;
   const uint64_t t0 = (uint64_t)local * tpb;
   const uint64_t tend = std::min<uint64_t>((uint64_t)it.ntiles, t0 + tpb);
-  tile_pass<R>((C<R>*)it.state, geom, mats, tile, t0, 1, tend, red, sums, it.fresh);
+  tile_pass<R, uint32_t>((C<R>*)it.state, geom, mats, tile, t0, 1, tend, red, sums, it.fresh);
   if (threadIdx.x < kAcc) part[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = sums[threadIdx.x];
 }
 
@@ -267,7 +268,7 @@ class RoundEngine {
     j->slots.assign(bti::kQMax, QOp());
     bti::Program prog{j->circ->ops.data(), j->circ->ntop, j->circ->pool.data(), j->circ->chan.data(),
                       j->circ->chanw.data(), j->nbits};
-    j->ip->init(prog, j->slots.data(), tile_T(j->nbits), 3);
+    j->ip->init(prog, j->slots.data(), tile_T(j->nbits), coalesce_bits(prec_));
     j->ip->seed = j->seed;
     j->ip->traj = j->traj;
     j->ip->fold = j->fold;

@@ -38,17 +38,22 @@ constexpr int kMaxPh = 28;
 // 13), complex128 16 (T = 12): 64 KiB tiles, registers <= 256 per lane.
 template <typename R>
 struct TileCfg;
+// WAVES: waves per SIMD the kernels are compiled for (launch-bounds
+// minimum): complex64 runs two 64 KiB-tile workgroups per CU (registers
+// <= 256), complex128 one (its tile + matrices exceed half the LDS).
 template <>
 struct TileCfg<float> {
   static constexpr int NT = 256;
   static constexpr int LOGNT = 8;
   static constexpr int RB = 5;
+  static constexpr int WAVES = 2;
 };
 template <>
 struct TileCfg<double> {
   static constexpr int NT = 256;
   static constexpr int LOGNT = 8;
   static constexpr int RB = 4;
+  static constexpr int WAVES = 1;
 };
 
 // An operator of a pass.  k <= 2: applied in registers, `mask` = register
@@ -73,6 +78,7 @@ struct RPhase {
 
 struct PassGeom {
   int32_t T, nrb, tb, ncomp, write, need_norm, red, red_k, red_swap, nph, nops, nmats;
+  int32_t hist_fast;         // RED_HIST with no histogram bit in the store layout's registers
   int32_t tile[bti::kMaxT];  // local bit -> global bit
   int32_t comp[40];          // global bits outside the tile, ascending
   int32_t nrun;              // tile index -> global base: runs of consecutive outside bits
@@ -155,24 +161,71 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
     if (nb > 4) return BT_ERR_UNSUPPORTED;
     lk[o] = nb;
   }
-  // 2. phases: consecutive operators whose bits fit in the registers
-  int nph = 0;
-  uint32_t cur = 0;  // local-bit mask of the open phase
-  int ph_of[bti::kPMax];
-  uint32_t ph_mask[kMaxPh];
-  for (int o = 0; o < plan.nops; o++) {
-    uint32_t m = 0;
+  // 2. phases: sets of operators whose register bits fit together.
+  //    Operators on disjoint qubits commute, so a phase may take any
+  //    operator whose predecessors (earlier operators sharing a qubit) are
+  //    already placed.  Greedy: each phase is grown from the seed that lets
+  //    it take the most operators; within a phase the original order holds.
+  const int nops = plan.nops;
+  uint32_t opm[bti::kPMax];   // register bits an operator needs (k <= 2)
+  uint32_t pred[bti::kPMax];  // operators that must run first
+  for (int o = 0; o < nops; o++) {
+    opm[o] = 0;
     if (lk[o] <= 2)
-      for (int i = 0; i < lk[o]; i++) m |= 1u << lbits[o][i];
-    if (nph == 0 || popc32(cur | m) > nrb) {
-      if (nph >= kMaxPh) return BT_ERR_UNSUPPORTED;
-      ph_mask[nph] = 0;
-      nph++;
-      cur = 0;
+      for (int i = 0; i < lk[o]; i++) opm[o] |= 1u << lbits[o][i];
+    const QOp& q = slots[plan.qidx[o]];
+    uint64_t to = q.ctrl;
+    for (int i = 0; i < q.k; i++) to |= 1ull << q.bits[i];
+    pred[o] = 0;
+    for (int a = 0; a < o; a++) {
+      const QOp& qa = slots[plan.qidx[a]];
+      uint64_t ta = qa.ctrl;
+      for (int i = 0; i < qa.k; i++) ta |= 1ull << qa.bits[i];
+      if (ta & to) pred[o] |= 1u << a;
     }
-    cur |= m;
-    ph_mask[nph - 1] = cur;
-    ph_of[o] = nph - 1;
+  }
+  int nph = 0;
+  uint32_t cur = 0;  // register-bit mask of the last phase
+  int ph_of[bti::kPMax];
+  int exec[bti::kPMax];  // execution order
+  int ne = 0;
+  uint32_t ph_mask[kMaxPh];
+  uint32_t done = 0;
+  const uint32_t all = nops >= 32 ? 0xffffffffu : (1u << nops) - 1u;
+  while (done != all) {
+    uint32_t best = 0, best_m = 0;
+    int best_n = -1;
+    for (int sd = 0; sd < nops; sd++) {
+      if (((done >> sd) & 1) || (pred[sd] & ~done)) continue;
+      uint32_t sel = 1u << sd, m = opm[sd];
+      int n = 1;
+      for (bool grew = true; grew;) {
+        grew = false;
+        for (int o = 0; o < nops; o++) {
+          if (((done | sel) >> o) & 1) continue;
+          if (pred[o] & ~(done | sel)) continue;
+          if (popc32(m | opm[o]) > nrb) continue;
+          sel |= 1u << o;
+          m |= opm[o];
+          n++;
+          grew = true;
+        }
+      }
+      if (n > best_n) {
+        best_n = n;
+        best = sel;
+        best_m = m;
+      }
+    }
+    if (best_n <= 0 || nph >= kMaxPh) return BT_ERR_UNSUPPORTED;
+    for (int o = 0; o < nops; o++)
+      if ((best >> o) & 1) {
+        ph_of[o] = nph;
+        exec[ne++] = o;
+      }
+    ph_mask[nph++] = best_m;
+    cur = best_m;
+    done |= best;
   }
   uint32_t redm = 0;
   if (plan.red == bti::RED_RHO) {
@@ -212,17 +265,47 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
       for (int j = 0; j < kRB; j++) g->io_in[j] = g->ph[0].rb[j];
     if (nph > 0 && !(ph_mask[nph - 1] & low))
       for (int j = 0; j < kRB; j++) g->io_out[j] = g->ph[nph - 1].rb[j];
+    // a histogram is cheapest when its bits stay off the store layout's
+    // registers: each thread's 2^RB amplitudes then share one bin per tile
+    g->hist_fast = 0;
+    if (plan.red == bti::RED_HIST) {
+      uint32_t lm = 0;
+      for (int i = 0; i < plan.red_k; i++)
+        if (rank[plan.red_bits[i]] >= 0) lm |= 1u << rank[plan.red_bits[i]];
+      uint32_t om = 0;
+      for (int j = 0; j < kRB; j++)
+        if (g->io_out[j] >= 0) om |= 1u << g->io_out[j];
+      if (om & lm) {
+        int hb[kRB], m = 0;
+        for (int b = T - 1; b >= 0 && m < nrb; b--)
+          if (!((lm >> b) & 1)) hb[m++] = b;
+        if (m == nrb) {
+          for (int i = 0; i < m / 2; i++) {
+            const int t = hb[i];
+            hb[i] = hb[m - 1 - i];
+            hb[m - 1 - i] = t;
+          }
+          set_layout(g->io_out, hb, m);
+          om = 0;
+          for (int j = 0; j < kRB; j++)
+            if (g->io_out[j] >= 0) om |= 1u << g->io_out[j];
+        }
+      }
+      g->hist_fast = (om & lm) ? 0 : 1;
+    }
   }
-  // 4. operators in register positions, matrices permuted to the ascending
-  //    position convention (index bit j <-> j-th lowest position)
+  // 4. operators in execution order, in register positions, matrices
+  //    permuted to the ascending position convention (index bit j <-> j-th
+  //    lowest position)
   int moff = 0;
   g->nops = plan.nops;
-  for (int o = 0; o < plan.nops; o++) {
+  for (int e = 0; e < plan.nops; e++) {
+    const int o = exec[e];
     const QOp& q = slots[plan.qidx[o]];
     const RPhase& ph = g->ph[ph_of[o]];
-    if (ph.nops == 0) g->ph[ph_of[o]].op0 = o;
+    if (ph.nops == 0) g->ph[ph_of[o]].op0 = e;
     g->ph[ph_of[o]].nops++;
-    ROp& r = g->ops[o];
+    ROp& r = g->ops[e];
     const int k2 = lk[o], D2 = 1 << k2;
     r.k = k2;
     int pos[kRB];  // register position of lbits[o][i] (k <= 2), else the operand order
@@ -461,27 +544,44 @@ __host__ __device__ __forceinline__ void apply_lds(C<R>* lds, int T, const ROp& 
 }
 
 #if defined(__HIPCC__)
-// Block-wide sum of M per-thread values, added to sums[0..M) (LDS) in a
-// fixed order (xor butterfly in each wavefront, then waves in order).
-template <int M, int NT>
-__device__ __forceinline__ void block_add(const double* v, double* red /*LDS [NT/64][M]*/, double* sums) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < M; j++) {
-    double x = v[j];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if (lane == 0) red[wave * M + j] = x;
-  }
-  __syncthreads();
-  if (threadIdx.x < M) {
-    double t = 0;
-    for (int w = 0; w < NT / 64; w++) t += red[w * M + threadIdx.x];
-    sums[threadIdx.x] += t;
-  }
-  __syncthreads();
+// A double moved across lanes by a DPP pattern (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wavefront in a fixed order (xor-1 and xor-2
+// within quads, then the half-row and row mirrors, then the four rows in
+// order); the same value in every lane.
+__device__ __forceinline__ double wave_sum(double x) {
+  x += dpp_f64<0xb1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4e>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
+}
+
+// Add M per-thread values into this wavefront's accumulator slots
+// (wred[0..M), LDS): one writer lane, tiles in order, no barrier.
+template <int M>
+__device__ __forceinline__ void wave_accumulate(const double* v, double* wred) {
+  const bool lead = (threadIdx.x & 63) == 0;
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    const double t = wave_sum(v[j]);
+    if (lead) wred[j] += t;
+  }
+}
 #endif
 
 // Local index of thread t's register r under a layout: thread bits are the
@@ -501,21 +601,50 @@ __host__ __device__ __forceinline__ uint32_t thread_base(const int8_t* rb, int T
   return base;
 }
 
+// A value the whole workgroup holds in common, moved to a scalar register
+// on the device (identity on the host).
+__host__ __device__ __forceinline__ uint32_t uni32(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_readfirstlane(x);
+#else
+  return x;
+#endif
+}
+
+// LDS slots of a layout.  The swizzle is linear over XOR, so the slot of
+// (thread base | register offset) is swz(base) ^ XOR of swz(bit) over the
+// register's bits: one VALU XOR per register against workgroup-uniform
+// (scalar) terms.
 template <typename R>
-__host__ __device__ __forceinline__ void regs_to_lds(const C<R>* v, C<R>* lds, const int8_t* lay, int T, uint32_t tid,
-                                            bool active, int nvalid_mask) {
-  constexpr int NR = 1 << TileCfg<R>::RB;
-  const uint32_t b0 = thread_base(lay, T, tid);
-  uint32_t m[kRB];
+struct LdsSlots {
+  uint32_t s0;        // swz(thread base), per thread
+  uint32_t sj[kRB];   // swz(1 << register bit j), uniform
+  __host__ __device__ __forceinline__ LdsSlots(const int8_t* lay, int T, uint32_t tid) {
+    s0 = swz<R>(thread_base(lay, T, tid));
 #pragma unroll
-  for (int j = 0; j < kRB; j++) m[j] = lay[j] >= 0 ? 1u << lay[j] : 0u;
-#pragma unroll
-  for (int r = 0; r < NR; r++) {
-    uint32_t l = b0;
+    for (int j = 0; j < kRB; j++) sj[j] = uni32(lay[j] >= 0 ? swz<R>(1u << lay[j]) : 0u);
+  }
+  __host__ __device__ __forceinline__ uint32_t slot(int r) const {
+    uint32_t u = 0;
 #pragma unroll
     for (int j = 0; j < kRB; j++)
-      if ((r >> j) & 1) l |= m[j];
-    if (active && !(r & nvalid_mask)) lds[swz<R>(l)] = v[r];
+      if ((r >> j) & 1) u ^= sj[j];
+    return s0 ^ u;
+  }
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ void regs_to_lds(const C<R>* v, C<R>* lds, const int8_t* lay, int T, uint32_t tid,
+                                                     bool active, int nvalid_mask) {
+  constexpr int NR = 1 << TileCfg<R>::RB;
+  const LdsSlots<R> S(lay, T, tid);
+  if (active && nvalid_mask == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) lds[S.slot(r)] = v[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      if (active && !(r & nvalid_mask)) lds[S.slot(r)] = v[r];
   }
 }
 
@@ -525,33 +654,27 @@ __host__ __device__ __forceinline__ void regs_to_lds(const C<R>* v, C<R>* lds, c
 template <typename R>
 __host__ __device__ __forceinline__ void lds_to_regs(C<R>* v, const C<R>* lds, const int8_t* lay, int T, uint32_t ltid) {
   constexpr int NR = 1 << TileCfg<R>::RB;
-  const uint32_t b0 = thread_base(lay, T, ltid);
-  uint32_t m[kRB];
+  const LdsSlots<R> S(lay, T, ltid);
 #pragma unroll
-  for (int j = 0; j < kRB; j++) m[j] = lay[j] >= 0 ? 1u << lay[j] : 0u;
-#pragma unroll
-  for (int r = 0; r < NR; r++) {
-    uint32_t l = b0;
-#pragma unroll
-    for (int j = 0; j < kRB; j++)
-      if ((r >> j) & 1) l |= m[j];
-    v[r] = lds[swz<R>(l)];
-  }
+  for (int r = 0; r < NR; r++) v[r] = lds[S.slot(r)];
 }
 
-// Per-thread constants of a pass: global offsets of the load / store
-// layouts.  Threads >= 2^tb are inactive; their loads mirror thread
-// tid mod 2^tb, so every address is inside the state.
+// Per-thread constants of a pass: offsets (in amplitudes, from the tile's
+// first amplitude) of the load / store layouts.  O = uint32_t when every
+// tile bit is below 32 (states of up to 2^32 amplitudes), else uint64_t.
+// Threads >= 2^tb are inactive; their loads mirror thread tid mod 2^tb, so
+// every address is inside the state.
+template <typename O>
 struct ThreadIO {
-  uint64_t gin[kRB], gout[kRB];
-  uint64_t gthr_in, gthr_out;
+  O gin[kRB], gout[kRB];
+  O gthr_in, gthr_out;
   uint32_t tid, ltid;
   bool active;
   int nvalid_mask;  // register bits beyond nrb are phantom
 };
 
-template <typename R>
-__host__ __device__ __forceinline__ void thread_io(const PassGeom& g, uint32_t tid, ThreadIO& io) {
+template <typename R, typename O>
+__host__ __device__ __forceinline__ void thread_io(const PassGeom& g, uint32_t tid, ThreadIO<O>& io) {
   constexpr int NR = 1 << TileCfg<R>::RB;
   io.tid = tid;
   io.active = tid < (1u << g.tb);
@@ -559,16 +682,39 @@ __host__ __device__ __forceinline__ void thread_io(const PassGeom& g, uint32_t t
   io.nvalid_mask = (NR - 1) & ~((1 << g.nrb) - 1);
 #pragma unroll
   for (int j = 0; j < kRB; j++) {
-    io.gin[j] = g.io_in[j] >= 0 ? 1ull << g.tile[g.io_in[j]] : 0;
-    io.gout[j] = g.io_out[j] >= 0 ? 1ull << g.tile[g.io_out[j]] : 0;
+    io.gin[j] = g.io_in[j] >= 0 ? (O)1 << g.tile[g.io_in[j]] : 0;
+    io.gout[j] = g.io_out[j] >= 0 ? (O)1 << g.tile[g.io_out[j]] : 0;
   }
   io.gthr_in = 0;
   io.gthr_out = 0;
   const uint32_t bi = thread_base(g.io_in, g.T, io.ltid), bo = thread_base(g.io_out, g.T, io.ltid);
   for (int b = 0; b < g.T; b++) {
-    if ((bi >> b) & 1) io.gthr_in |= 1ull << g.tile[b];
-    if ((bo >> b) & 1) io.gthr_out |= 1ull << g.tile[b];
+    if ((bi >> b) & 1) io.gthr_in |= (O)1 << g.tile[b];
+    if ((bo >> b) & 1) io.gthr_out |= (O)1 << g.tile[b];
   }
+}
+
+// Hide a value's origin from the optimiser (no instruction is emitted), so
+// the per-register offsets built from it are recomputed per tile instead of
+// being hoisted out of the tile loop as 64 live registers.
+template <typename T>
+__host__ __device__ __forceinline__ void opaque(T& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x));
+#else
+  (void)x;
+#endif
+}
+
+template <typename O>
+__host__ __device__ __forceinline__ void opaque_io(ThreadIO<O>& io) {
+#pragma unroll
+  for (int j = 0; j < kRB; j++) {
+    opaque(io.gin[j]);
+    opaque(io.gout[j]);
+  }
+  opaque(io.gthr_in);
+  opaque(io.gthr_out);
 }
 
 // global index of tile t's first amplitude
@@ -582,67 +728,85 @@ __host__ __device__ __forceinline__ uint64_t tile_base(const PassGeom& g, uint64
   return base;
 }
 
-// Load a tile's registers (io_in layout), scaled.  The loads are
-// unconditional and straight-line (phantom registers alias a valid
-// address) so they issue back to back; what is not ours is zeroed after.
-// fresh != 0: the state is |0...0> and is not read.
-template <typename R>
-__host__ __device__ __forceinline__ void load_regs(C<R>* v, const C<R>* __restrict__ state, const ThreadIO& io,
-                                                   uint64_t base, R scale, int fresh) {
+// Issue the loads of a tile's registers (io_in layout) into raw[].  The
+// loads are unconditional and straight-line (phantom registers alias a
+// valid address) so they are all in flight at once; the device loop issues
+// them for the next tile before it computes on the current one.
+template <typename R, typename O>
+__host__ __device__ __forceinline__ void fetch_regs(C<R>* raw, const C<R>* __restrict__ state, const ThreadIO<O>& io,
+                                                    uint64_t base) {
   constexpr int NR = 1 << TileCfg<R>::RB;
-  if (fresh) {
+  const C<R>* __restrict__ tp = state + base;
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-      uint64_t gi = base | io.gthr_in;
+  for (int r = 0; r < NR; r++) {
+    O off = io.gthr_in;
+#pragma unroll
+    for (int j = 0; j < kRB; j++)
+      if ((r >> j) & 1) off |= io.gin[j];
+    raw[r] = tp[off];
+  }
+}
+
+// Registers of a tile from its fetched values: scaled, and zero where the
+// register is phantom or the thread inactive.  fresh != 0: the state is
+// |0...0> and nothing was fetched.
+template <typename R, typename O>
+__host__ __device__ __forceinline__ void take_regs(C<R>* v, const C<R>* raw, const ThreadIO<O>& io, uint64_t base,
+                                                   R scale, int fresh) {
+  constexpr int NR = 1 << TileCfg<R>::RB;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const bool ok = io.active && !(r & io.nvalid_mask);
+    if (fresh) {
+      O off = io.gthr_in;
 #pragma unroll
       for (int j = 0; j < kRB; j++)
-        if ((r >> j) & 1) gi |= io.gin[j];
-      const bool ok = io.active && !(r & io.nvalid_mask);
-      v[r] = cmake<C<R>>(ok && gi == 0 ? scale : (R)0, (R)0);
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-      uint64_t gi = base | io.gthr_in;
-#pragma unroll
-      for (int j = 0; j < kRB; j++)
-        if ((r >> j) & 1) gi |= io.gin[j];
-      v[r] = state[gi];
-    }
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-      const bool ok = io.active && !(r & io.nvalid_mask);
-      v[r].x = ok ? v[r].x * scale : (R)0;
-      v[r].y = ok ? v[r].y * scale : (R)0;
+        if ((r >> j) & 1) off |= io.gin[j];
+      v[r] = cmake<C<R>>(ok && base == 0 && off == 0 ? scale : (R)0, (R)0);
+    } else {
+      v[r].x = ok ? raw[r].x * scale : (R)0;
+      v[r].y = ok ? raw[r].y * scale : (R)0;
     }
   }
 }
 
 // Store a tile's registers (io_out layout) and accumulate FP64 norm^2 into
-// acc[0] and the histogram over g.red_bits into acc[1..16].
-template <typename R>
+// acc[0] and the histogram over g.red_bits into acc[1..2^red_k].
+template <typename R, typename O>
 __host__ __device__ __forceinline__ void store_regs(const C<R>* v, C<R>* __restrict__ state, const PassGeom& g,
-                                                    const ThreadIO& io, uint64_t base, double* acc) {
+                                                    const ThreadIO<O>& io, uint64_t base, double* acc) {
   constexpr int NR = 1 << TileCfg<R>::RB;
+  C<R>* __restrict__ tp = state + base;
+  const bool slow_hist = g.red == bti::RED_HIST && !g.hist_fast;
+  double s = 0.0;
 #pragma unroll
   for (int r = 0; r < NR; r++) {
-    uint64_t gi = base | io.gthr_out;
+    O off = io.gthr_out;
 #pragma unroll
     for (int j = 0; j < kRB; j++)
-      if ((r >> j) & 1) gi |= io.gout[j];
+      if ((r >> j) & 1) off |= io.gout[j];
     if (io.active && !(r & io.nvalid_mask)) {
-      if (g.write) state[gi] = v[r];
+      if (g.write) tp[off] = v[r];
       if (g.need_norm) {
         const double w = (double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y;
-        acc[0] += w;
-        if (g.red == bti::RED_HIST) {
+        s += w;
+        if (slow_hist) {
           int bin = 0;
+          const uint64_t gi = base | (uint64_t)off;
           for (int j = 0; j < g.red_k; j++) bin |= (int)((gi >> g.red_bits[j]) & 1ull) << j;
 #pragma unroll
           for (int b = 0; b < 16; b++) acc[1 + b] += (bin == b) ? w : 0.0;
         }
       }
     }
+  }
+  acc[0] += s;
+  if (g.red == bti::RED_HIST && g.hist_fast) {  // one bin for all of this thread's amplitudes
+    const uint64_t gi = base | (uint64_t)io.gthr_out;
+    int bin = 0;
+    for (int j = 0; j < g.red_k; j++) bin |= (int)((gi >> g.red_bits[j]) & 1ull) << j;
+#pragma unroll
+    for (int b = 0; b < 16; b++) acc[1 + b] += (bin == b) ? s : 0.0;
   }
 }
 
@@ -651,6 +815,13 @@ __host__ __device__ __forceinline__ int rho_reg_mask(const PassGeom& g) {
 }
 
 #if defined(__HIPCC__)
+// A value the whole workgroup holds in common, moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Copy a pass geometry into LDS: the header, the used phases and operators.
 __device__ __forceinline__ void load_geom(const PassGeom* __restrict__ src, PassGeom* dst, int tid, int nt) {
   constexpr int hdr = (int)(offsetof(PassGeom, ph) / 4);
@@ -681,10 +852,12 @@ __device__ __forceinline__ double sum_blocks(const double* __restrict__ p, uint3
 
 // One pass over tiles t = t0, t0 + tstep, ... < tend.  Reductions are added
 // to sums[0..kAcc) (LDS, zeroed by the caller): [0] norm^2, [1..16]
-// histogram, [17..32] packed rho.  lds holds 2^T elements.
+// histogram, [17..32] packed rho; each wavefront accumulates its tiles in
+// red[wave * kAcc + j] (LDS, NT/64 * kAcc doubles) and the waves are summed
+// in order at the end.  lds holds 2^T elements.
 // fresh != 0: the state is |0...0> and is not read (its first pass).
 // The host emulation of this loop (tests/emu) follows it step for step.
-template <typename R>
+template <typename R, typename O>
 __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGeom& g, const C<R>* mats, C<R>* lds,
                                           uint64_t t0, uint64_t tstep, uint64_t tend, double* red, double* sums,
                                           int fresh = 0) {
@@ -692,12 +865,19 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
   constexpr int NR = 1 << TileCfg<R>::RB;
   const int T = g.T;
   const R scale = (R)g.scale;
-  ThreadIO io;
-  thread_io<R>(g, threadIdx.x, io);
-  C<R> v[NR];
+  double* wred = red + (threadIdx.x >> 6) * kAcc;
+  for (int i = threadIdx.x; i < (NT / 64) * kAcc; i += NT) red[i] = 0.0;
+  __syncthreads();
+  ThreadIO<O> io;
+  thread_io<R, O>(g, threadIdx.x, io);
+  C<R> v[NR], nx[NR];
+  if (!fresh && t0 < tend) fetch_regs<R, O>(nx, state, io, uniform_u64(tile_base(g, t0)));
   for (uint64_t t = t0; t < tend; t += tstep) {
-    const uint64_t base = tile_base(g, t);
-    load_regs<R>(v, state, io, base, scale, fresh);
+    opaque_io(io);
+    const uint64_t base = uniform_u64(tile_base(g, t));
+    take_regs<R, O>(v, nx, io, base, scale, fresh);
+    // the next tile's loads fly while this one computes (tiles are disjoint)
+    if (!fresh && t + tstep < tend) fetch_regs<R, O>(nx, state, io, uniform_u64(tile_base(g, t + tstep)));
     int8_t cur[kRB];
 #pragma unroll
     for (int j = 0; j < kRB; j++) cur[j] = g.io_in[j];
@@ -738,16 +918,31 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
 #pragma unroll
         for (int j = 0; j < 16; j++) racc[j] = 0.0;
         if (io.active) rho_mask<R>(rho_reg_mask(g), v, io.nvalid_mask, racc);
-        block_add<16, NT>(racc, red, sums + 17);
+        wave_accumulate<16>(racc, wred + 17);
       }
     }
     double acc[17];
 #pragma unroll
     for (int j = 0; j < 17; j++) acc[j] = 0.0;
-    store_regs<R>(v, state, g, io, base, acc);
-    if (g.red == bti::RED_HIST) block_add<17, NT>(acc, red, sums);
-    else if (g.need_norm) block_add<1, NT>(acc, red, sums);
+    store_regs<R, O>(v, state, g, io, base, acc);
+    if (g.red == bti::RED_HIST) {
+      switch (g.red_k) {  // norm + 2^red_k bins
+        case 1: wave_accumulate<3>(acc, wred); break;
+        case 2: wave_accumulate<5>(acc, wred); break;
+        case 3: wave_accumulate<9>(acc, wred); break;
+        default: wave_accumulate<17>(acc, wred); break;
+      }
+    } else if (g.need_norm) {
+      wave_accumulate<1>(acc, wred);
+    }
   }
+  __syncthreads();
+  if (threadIdx.x < kAcc) {
+    double t = 0.0;
+    for (int w = 0; w < NT / 64; w++) t += red[w * kAcc + threadIdx.x];
+    sums[threadIdx.x] += t;
+  }
+  __syncthreads();
 }
 #endif
 

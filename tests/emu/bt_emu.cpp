@@ -61,13 +61,22 @@ static int g_exec = 0;
 // per tile, operators applied through LDS
 static int64_t g_gstat[9];
 
+static FILE* g_geom_dump = nullptr;
+
 extern "C" void emu_set_exec(int mode) { g_exec = mode; }
+// Append every register-tile pass geometry (PassGeom, then nmats complex
+// entries of the pass precision) to a file; NULL closes it.
+extern "C" int emu_dump_geometry(const char* path) {
+  if (g_geom_dump) fclose(g_geom_dump);
+  g_geom_dump = path ? fopen(path, "wb") : nullptr;
+  return path && !g_geom_dump ? -1 : 0;
+}
 extern "C" void emu_geom_stats(int64_t* out, int reset) {
   for (int i = 0; i < 9; i++) out[i] = g_gstat[i];
   if (reset) memset(g_gstat, 0, sizeof g_gstat);
 }
 
-template <typename R>
+template <typename R, typename O>
 static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& plan, const bti::QOp* slots,
                          bti::PassResult* res, int fresh) {
   using namespace bt;
@@ -82,6 +91,10 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
     return rc;
   }
   g_gstat[0]++;
+  if (g_geom_dump) {
+    fwrite(&g, sizeof g, 1, g_geom_dump);
+    fwrite(mats.data(), sizeof(C<R>), (size_t)g.nmats, g_geom_dump);
+  }
   g_gstat[1] += g.nops;
   for (int o = 0; o < g.nops; o++) g_gstat[1 + g.ops[o].k]++;
   g_gstat[6] += g.nph;
@@ -90,13 +103,17 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
   const uint64_t ntiles = 1ull << (nbits - T);
   std::vector<C<R>> lds((size_t)1 << T);
   std::vector<std::array<C<R>, NR>> v(NT);
-  std::vector<ThreadIO> io(NT);
-  for (int tid = 0; tid < NT; tid++) thread_io<R>(g, tid, io[tid]);
+  std::vector<ThreadIO<O>> io(NT);
+  for (int tid = 0; tid < NT; tid++) thread_io<R, O>(g, tid, io[tid]);
   double sums[kAcc];
   for (int j = 0; j < kAcc; j++) sums[j] = 0.0;
   for (uint64_t t = 0; t < ntiles; t++) {
     const uint64_t base = tile_base(g, t);
-    for (int tid = 0; tid < NT; tid++) load_regs<R>(v[tid].data(), st.data(), io[tid], base, scale, fresh);
+    for (int tid = 0; tid < NT; tid++) {
+      std::array<C<R>, NR> raw{};
+      if (!fresh) fetch_regs<R, O>(raw.data(), st.data(), io[tid], base);
+      take_regs<R, O>(v[tid].data(), raw.data(), io[tid], base, scale, fresh);
+    }
     int8_t cur[kRB];
     for (int j = 0; j < kRB; j++) cur[j] = g.io_in[j];
     for (int p = 0; p <= g.nph; p++) {
@@ -141,7 +158,7 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
     for (int tid = 0; tid < NT; tid++) {
       double acc[17];
       for (int j = 0; j < 17; j++) acc[j] = 0.0;
-      store_regs<R>(v[tid].data(), st.data(), g, io[tid], base, acc);
+      store_regs<R, O>(v[tid].data(), st.data(), g, io[tid], base, acc);
       if (g.red == bti::RED_HIST)
         for (int j = 0; j < 17; j++) sums[j] += acc[j];
       else if (g.need_norm)
@@ -173,7 +190,9 @@ static int run_tiles(bti::Interp& ip, const bti::QOp* slots, int nbits, std::vec
       return BT_ERR_STATE;
     }
     memset(&res, 0, sizeof res);
-    const int rc = tile_pass_emu<R>(st, nbits, plan, slots, &res, fresh);
+    // offsets as the device: 32-bit up to 2^32 amplitudes
+    const int rc = nbits <= 32 ? tile_pass_emu<R, uint32_t>(st, nbits, plan, slots, &res, fresh)
+                               : tile_pass_emu<R, uint64_t>(st, nbits, plan, slots, &res, fresh);
     if (rc) return rc;
     if (plan.write) fresh = 0;
   }
