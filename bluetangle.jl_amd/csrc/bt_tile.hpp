@@ -253,17 +253,19 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
     g->ph[p].nops = 0;
   }
   g->nph = nph;
-  // load/store layouts: the top local bits in registers, unless a phase's own
-  // layout keeps the lowest min(3, tb) local bits on the threads
+  // load/store layouts: the top local bits in registers, unless the first /
+  // last phase's layout holds at most two of the lowest four local bits in
+  // registers (the wavefront still reads whole 128 B lines within a few
+  // instructions, and a relayout costs more than the extra requests)
   {
     int bits[kRB], n = 0;
     for (int b = T - nrb; b < T; b++) bits[n++] = b;
     set_layout(g->io_in, bits, n);
     set_layout(g->io_out, bits, n);
-    const uint32_t low = (1u << (g->tb < 3 ? g->tb : 3)) - 1u;
-    if (nph > 0 && !(ph_mask[0] & low))
+    const uint32_t low = (1u << (g->tb < 4 ? g->tb : 4)) - 1u;
+    if (nph > 0 && popc32(ph_mask[0] & low) <= 2)
       for (int j = 0; j < kRB; j++) g->io_in[j] = g->ph[0].rb[j];
-    if (nph > 0 && !(ph_mask[nph - 1] & low))
+    if (nph > 0 && popc32(ph_mask[nph - 1] & low) <= 2)
       for (int j = 0; j < kRB; j++) g->io_out[j] = g->ph[nph - 1].rb[j];
     // a histogram is cheapest when its bits stay off the store layout's
     // registers: each thread's 2^RB amplitudes then share one bin per tile
