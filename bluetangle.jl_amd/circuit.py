@@ -215,6 +215,21 @@ class Circuit:
                                        ctypes.byref(nout), ctypes.byref(st)))
         return outs[: nout.value].tolist(), st.as_dict()
 
+    def run_fused(self, state: StateVector, max_k=5, norm_every=0):
+        """The (noiseless, gate-only) circuit fused into dense blocks of at
+        most max_k bits, one pass per block (complex64, max_k=5: FP32 MFMA).
+        Returns (stats, FP64 norms after every norm_every blocks)."""
+        nb = ctypes.c_int64()
+        st = L.RunStats()
+        cap = 4096 if norm_every > 0 else 0
+        norms = np.zeros(max(cap, 1), dtype=np.float64)
+        L.check(L.lib().bt_run_fused(state.handle, self.handle, max_k, norm_every, L.dptr(norms) if cap else None,
+                                     cap, ctypes.byref(nb), ctypes.byref(st)))
+        d = st.as_dict()
+        d["blocks"] = nb.value
+        nn = min(cap, nb.value // norm_every) if norm_every > 0 else 0
+        return d, norms[:nn].tolist()
+
     def run_dm(self, rho: DensityMatrix):
         st = L.RunStats()
         L.check(L.lib().bt_run_circuit_dm(rho.handle, self.handle, ctypes.byref(st)))

@@ -1,0 +1,285 @@
+// Dense 5-qubit blocks on FP32 MFMA (complex64 state vectors), and the fused
+// circuit executor built on them.
+//
+// A 32 x 32 complex block acting on 5 bits is, per group of 32 amplitudes,
+// the real product y = A x with A = [[Re M, -Im M], [Im M, Re M]] (64 x 64)
+// and x = (Re, Im) interleaved.  k_fused5 streams the state in tiles of
+// 2^12 amplitudes (the 5 block bits + the 7 lowest other bits): a coalesced
+// load into LDS, then every wavefront multiplies 32 of the tile's 128 groups
+// with v_mfma_f32_32x32x2f32 (A resident in registers, the groups as the 32
+// columns: lanes 0-31 supply the real and lanes 32-63 the imaginary part of
+// each component), writes its results back over its own groups, and the tile
+// is stored coalesced.  Per amplitude: 128 real MACs = 16 flop/byte of HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bt_circuit.hpp"
+#include "bt_fuse.hpp"
+#include "bt_internal.hpp"
+
+namespace bt {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kF5T = 12;  // tile bits
+constexpr int kF5N = 1 << kF5T;
+
+struct F5Args {
+  float2* s;
+  uint64_t ntiles;
+  int32_t tile[kF5T];   // local bit -> global bit, ascending
+  int32_t nrun;         // tile index -> global base: runs of consecutive outside bits
+  int32_t run_dst[40], run_len[40];
+  int32_t vloc[7];      // local positions of the group-index bits, ascending
+  int32_t cloc[5];      // local position of operand bit i (i = 0 is the MSB)
+};
+
+// LDS slot of local index l: XOR-swizzle of the low 5 bits (8-byte elements)
+__device__ __forceinline__ uint32_t f5_swz(uint32_t l) { return l ^ (((l >> 5) ^ (l >> 10)) & 31u); }
+
+__global__ void __launch_bounds__(256, 2) k_fused5(F5Args a, const float* __restrict__ Alane) {
+  __shared__ float2 lds[kF5N];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // A operand of instruction i, row block rb: lane holds A[32 rb + lane % 32][2 i + lane / 32]
+  float a0[32], a1[32];
+#pragma unroll
+  for (int i = 0; i < 32; i++) {
+    a0[i] = Alane[i * 64 + lane];
+    a1[i] = Alane[(32 + i) * 64 + lane];
+  }
+  // this lane's group: g = 32 * wave + lane % 32 -> local base
+  const uint32_t grp = 32 * wave + (lane & 31);
+  uint32_t gb = 0;
+#pragma unroll
+  for (int j = 0; j < 7; j++)
+    if ((grp >> j) & 1) gb |= 1u << a.vloc[j];
+  const uint32_t s0 = f5_swz(gb);
+  // component c's local offset (operand 0 = MSB of c), swizzled (the swizzle is XOR-linear)
+  uint32_t cb[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) cb[i] = __builtin_amdgcn_readfirstlane(f5_swz(1u << a.cloc[i]));
+  // load / store: thread tid takes local indices tid + 256 i (i < 16)
+  uint64_t dlo = 0;
+#pragma unroll
+  for (int b = 0; b < 8; b++)
+    if ((tid >> b) & 1) dlo |= 1ull << a.tile[b];
+  uint64_t dhi[4];
+#pragma unroll
+  for (int b = 0; b < 4; b++) dhi[b] = 1ull << a.tile[8 + b];
+  const bool hi_half = lane >= 32;
+  for (uint64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    uint64_t base = 0, tt = t;
+    for (int r = 0; r < a.nrun; r++) {
+      base |= (tt & ((1ull << a.run_len[r]) - 1ull)) << a.run_dst[r];
+      tt >>= a.run_len[r];
+    }
+    float2 x[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      uint64_t gi = base | dlo;
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        if ((i >> b) & 1) gi |= dhi[b];
+      x[i] = a.s[gi];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) lds[f5_swz(tid + 256u * i)] = x[i];
+    __syncthreads();
+    float bv[32];
+#pragma unroll
+    for (int c = 0; c < 32; c++) {
+      uint32_t sc = 0;
+#pragma unroll
+      for (int i = 0; i < 5; i++)
+        if ((c >> (4 - i)) & 1) sc ^= cb[i];
+      const float2 e = lds[s0 ^ sc];
+      bv[c] = hi_half ? e.y : e.x;
+    }
+    floatx16 d0 = {}, d1 = {};
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[i], bv[i], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[i], bv[i], d1, 0, 0, 0);
+    }
+    // row m of block rb = 8 (j / 4) + 4 (lane / 32) + j % 4: (re, im) of output 16 rb + m / 2
+#pragma unroll
+    for (int j = 0; j < 16; j += 2) {
+      const int m = 8 * (j / 4) + (j % 4) + (hi_half ? 4 : 0);
+#pragma unroll
+      for (int rb = 0; rb < 2; rb++) {
+        const int c = 16 * rb + m / 2;
+        uint32_t sc = 0;
+#pragma unroll
+        for (int i = 0; i < 5; i++)
+          if ((c >> (4 - i)) & 1) sc ^= cb[i];
+        const floatx16& d = rb ? d1 : d0;
+        lds[s0 ^ sc] = make_float2(d[j], d[j + 1]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      uint64_t gi = base | dlo;
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        if ((i >> b) & 1) gi |= dhi[b];
+      a.s[gi] = lds[f5_swz(tid + 256u * i)];
+    }
+    __syncthreads();
+  }
+}
+
+// A (the real 64 x 64 form of a 32 x 32 complex128 column-major block) in
+// MFMA lane order: out[(rb * 32 + i) * 64 + lane] = A[32 rb + lane % 32][2 i + lane / 32].
+void fused5_lane_matrix(const double* mat, float* out) {
+  for (int rb = 0; rb < 2; rb++)
+    for (int i = 0; i < 32; i++)
+      for (int lane = 0; lane < 64; lane++) {
+        const int m = 32 * rb + lane % 32, k = 2 * i + lane / 32;
+        const int r = m / 2, qr = m % 2, c = k / 2, qc = k % 2;
+        const double re = mat[2 * (r + c * 32)], im = mat[2 * (r + c * 32) + 1];
+        const double v = (qr == 0) ? (qc == 0 ? re : -im) : (qc == 0 ? im : re);
+        out[(rb * 32 + i) * 64 + lane] = (float)v;
+      }
+}
+
+// psi <- M psi on 5 bits (bits[0] = MSB); d_alane: M in MFMA lane order on
+// the device (4096 floats).  Enqueued on the state's stream.
+int launch_fused5(bt_state_s* s, const int* bits, const float* d_alane) {
+  const int N = s->n;
+  if (s->kind != BT_SV || s->prec != BT_C64 || N < kF5T) return fail(BT_ERR_UNSUPPORTED, "fused5: c64 SV, N >= 12");
+  F5Args a;
+  memset(&a, 0, sizeof a);
+  a.s = (float2*)s->d;
+  uint64_t bm = 0;
+  for (int i = 0; i < 5; i++) {
+    if (bits[i] < 0 || bits[i] >= N) return fail(BT_ERR_ARG, "fused5: bit out of range");
+    bm |= 1ull << bits[i];
+  }
+  if (__builtin_popcountll(bm) != 5) return fail(BT_ERR_ARG, "fused5: repeated bit");
+  uint64_t tm = bm;
+  for (int b = 0; b < N && __builtin_popcountll(tm) < kF5T; b++) tm |= 1ull << b;
+  int n = 0;
+  int rank[64];
+  for (int b = 0; b < 64; b++) rank[b] = -1;
+  for (int b = 0; b < N; b++)
+    if ((tm >> b) & 1) {
+      rank[b] = n;
+      a.tile[n++] = b;
+    }
+  int nv = 0;
+  for (int l = 0; l < kF5T; l++)
+    if (!((bm >> a.tile[l]) & 1)) a.vloc[nv++] = l;
+  for (int i = 0; i < 5; i++) a.cloc[i] = rank[bits[i]];
+  a.nrun = 0;
+  int prev = -2;
+  for (int b = 0; b < N; b++)
+    if (!((tm >> b) & 1)) {
+      if (b != prev + 1) {
+        a.run_dst[a.nrun] = b;
+        a.run_len[a.nrun] = 0;
+        a.nrun++;
+      }
+      a.run_len[a.nrun - 1]++;
+      prev = b;
+    }
+  a.ntiles = 1ull << (N - kF5T);
+  const unsigned grid = (unsigned)std::min<uint64_t>(a.ntiles, (uint64_t)ctx().cus * 8);
+  hipLaunchKernelGGL(k_fused5, dim3(grid), dim3(256), 0, s->stream, a, d_alane);
+  BT_HIP(hipGetLastError());
+  return BT_OK;
+}
+
+}  // namespace bt
+
+using namespace bt;
+
+extern "C" int bt_apply_fused5(bt_state s, const double* mat, const int* bits) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (!s || !mat || !bits) return fail(BT_ERR_ARG, "null argument");
+  std::vector<float> al(64 * 64);
+  fused5_lane_matrix(mat, al.data());
+  float* d = nullptr;
+  BT_HIP(hipMalloc((void**)&d, sizeof(float) * al.size()));
+  rc = hipMemcpyAsync(d, al.data(), sizeof(float) * al.size(), hipMemcpyHostToDevice, s->stream) == hipSuccess
+           ? launch_fused5(s, bits, d)
+           : fail(BT_ERR_HIP, "fused5: matrix upload");
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  return rc;
+}
+
+extern "C" int bt_run_fused(bt_state s, bt_circuit c, int max_k, int norm_every, double* norms, int64_t nnorms,
+                            int64_t* nblocks, bt_run_stats* stats) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (!s || !c) return fail(BT_ERR_ARG, "null state or circuit");
+  if (s->kind != BT_SV) return fail(BT_ERR_ARG, "bt_run_fused: state vectors only");
+  if (s->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
+  const bool mfma = s->prec == BT_C64 && max_k == 5 && s->n >= kF5T;
+  std::vector<btf::Block> blocks;
+  std::string msg;
+  rc = btf::fuse_gates(c->h, max_k, mfma ? 5 : 0, &blocks, &msg);
+  if (rc) return fail(rc, msg);
+  if (nblocks) *nblocks = (int64_t)blocks.size();
+  // every 5-bit block's MFMA operand in one device buffer
+  float* d_al = nullptr;
+  float* h_al = nullptr;
+  const size_t per = 64 * 64;
+  if (mfma && !blocks.empty()) {
+    BT_HIP(hipHostMalloc((void**)&h_al, sizeof(float) * per * blocks.size()));
+    for (size_t b = 0; b < blocks.size(); b++)
+      fused5_lane_matrix((const double*)blocks[b].m.data(), h_al + per * b);
+    if (hipMalloc((void**)&d_al, sizeof(float) * per * blocks.size()) != hipSuccess) {
+      (void)hipHostFree(h_al);
+      return fail(BT_ERR_NOMEM, "bt_run_fused: matrix buffer");
+    }
+    if (hipMemcpyAsync(d_al, h_al, sizeof(float) * per * blocks.size(), hipMemcpyHostToDevice, s->stream) !=
+        hipSuccess)
+      rc = fail(BT_ERR_HIP, "bt_run_fused: matrix upload");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == BT_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(BT_ERR_HIP, "event");
+  if (rc == BT_OK) (void)hipEventRecord(e0, s->stream);
+  int64_t nn = 0, reads = 0;
+  for (size_t b = 0; rc == BT_OK && b < blocks.size(); b++) {
+    const btf::Block& blk = blocks[b];
+    if (mfma) {
+      rc = launch_fused5(s, blk.bits, d_al + per * b);
+    } else {
+      Placement p;
+      rc = make_placement(s->nbits, blk.k, blk.bits, 0, &p);
+      if (rc == BT_OK) rc = launch_apply(s, s->nbits, blk.k, (const double*)blk.m.data(), p, 1.0, false);
+    }
+    if (rc == BT_OK && norms && norm_every > 0 && (int64_t)(b + 1) % norm_every == 0 && nn < nnorms) {
+      rc = bt_norm2(s, norms + nn);
+      nn++;
+      reads++;
+    }
+  }
+  if (rc == BT_OK) {
+    (void)hipEventRecord(e1, s->stream);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(BT_ERR_HIP, "bt_run_fused: sync");
+  }
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    stats->passes = (int64_t)blocks.size();
+    stats->reduce_passes = reads;
+    stats->bytes = ((double)blocks.size() * 2.0 + (double)reads) * (double)s->len * (double)s->elem();
+    stats->launches = (int64_t)blocks.size();
+    float ms = 0;
+    if (rc == BT_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) stats->kernel_ms = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (d_al) (void)hipFree(d_al);
+  if (h_al) (void)hipHostFree(h_al);
+  return rc;
+}

@@ -176,6 +176,19 @@ int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t n
 int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
                             int64_t outcap, int64_t* nouts, double* final_states, bt_run_stats* stats);
 
+/* Dense 5-qubit block on FP32 MFMA: psi <- M psi on bits (bits[0] = MSB),
+ * M 32 x 32 complex128 column-major; complex64 state vectors with N >= 12.
+ * The fused form of hilbert(N, mat, ...) * state (hilbert.jl:18-159, 505)
+ * for a block of gates. */
+int bt_apply_fused5(bt_state s, const double* mat, const int* bits);
+/* A noiseless gate-only circuit (the plain apply loop, hilbert.jl:517-553)
+ * fused into dense blocks of at most max_k bits (2..5), one pass per block;
+ * complex64 with max_k = 5 runs every block on FP32 MFMA.  norm_every > 0:
+ * the FP64 norm after every norm_every blocks goes to norms[0..nnorms).
+ * *nblocks: blocks applied. */
+int bt_run_fused(bt_state s, bt_circuit c, int max_k, int norm_every, double* norms, int64_t nnorms, int64_t* nblocks,
+                 bt_run_stats* stats);
+
 /* Tile width (bits held in LDS per pass) of the circuit executors for a
  * precision; 0 restores the default (14 for BT_C64, 13 for BT_C128).  Set by
  * the autotuner.  Process-wide. */

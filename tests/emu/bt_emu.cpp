@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "bt_circuit.hpp"
+#include "bt_fuse.hpp"
 #include "bt_interp.hpp"
 #include "bt_tile.hpp"
 
@@ -300,4 +301,27 @@ extern "C" int emu_run(int nbits, const bt_ir_op* ops, int nops, int ntop, const
   stats[3] = ip.draw;
   stats[4] = ip.overflow;
   return BT_OK;
+}
+
+// The fusion planner of the fused executor: blocks (k, bits MSB first,
+// 2^k x 2^k complex128 column-major) written to out_k[b], out_bits[5 b ..],
+// out_mats (1024 complex per block).  Returns the block count or < 0.
+extern "C" int64_t emu_fuse(int nbits, const bt_ir_op* ops, int nops, int ntop, const double* pool, int64_t npool,
+                            int max_k, int pad_to, int* out_k, int* out_bits, double* out_mats, int64_t cap) {
+  btc::CircuitHost c;
+  int rc = btc::analyze(nbits, ops, nops, ntop, pool, npool, &c, &g_msg);
+  if (rc) return -rc;
+  std::vector<btf::Block> blocks;
+  rc = btf::fuse_gates(c, max_k, pad_to, &blocks, &g_msg);
+  if (rc) return -rc;
+  if ((int64_t)blocks.size() > cap) return -1000;
+  for (size_t b = 0; b < blocks.size(); b++) {
+    out_k[b] = blocks[b].k;
+    for (int i = 0; i < 5; i++) out_bits[5 * b + i] = blocks[b].bits[i];
+    for (size_t e = 0; e < blocks[b].m.size(); e++) {
+      out_mats[2 * (1024 * b + e)] = blocks[b].m[e].real();
+      out_mats[2 * (1024 * b + e) + 1] = blocks[b].m[e].imag();
+    }
+  }
+  return (int64_t)blocks.size();
 }

@@ -17,7 +17,8 @@ EMU_DIR = os.path.join(ROOT, "tests", "emu")
 EMU_SO = os.path.join(EMU_DIR, "libbt_emu.so")
 CSRC = os.path.join(ROOT, "bluetangle.jl_amd", "csrc")
 SRCS = [os.path.join(EMU_DIR, "bt_emu.cpp"), os.path.join(CSRC, "bt_interp.hpp"), os.path.join(CSRC, "bt_circuit.hpp"),
-        os.path.join(CSRC, "bt_tile.hpp"), os.path.join(CSRC, "bt_internal.hpp"), os.path.join(ROOT, "include", "bt.h")]
+        os.path.join(CSRC, "bt_tile.hpp"), os.path.join(CSRC, "bt_internal.hpp"), os.path.join(CSRC, "bt_fuse.hpp"),
+        os.path.join(ROOT, "include", "bt.h")]
 ROCM_INC = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "include")
 
 
@@ -180,3 +181,63 @@ def test_register_tile_pass_brickwork(emu, bt, mode):
         emu.emu_set_exec(0)
     assert outs == routs
     assert np.abs(s - r).max() < (1e-11 if mode == 1 else 1e-5)
+
+
+def _apply_block_dense(state, N, k, bits, m):
+    """psi <- M psi on bits (bits[0] = MSB of M's index), plain numpy."""
+    psi = state.copy()
+    D = 1 << k
+    idx = np.arange(2 ** N)
+    sub = np.zeros(2 ** N, dtype=np.int64)
+    for i, b in enumerate(bits[:k]):
+        sub |= ((idx >> b) & 1) << (k - 1 - i)
+    rest = idx.copy()
+    for b in bits[:k]:
+        rest &= ~(1 << b)
+    out = np.zeros_like(psi)
+    for r in range(D):
+        for c in range(D):
+            if m[r, c] == 0:
+                continue
+            sel = sub == c
+            tgt = rest[sel].copy()
+            for i, b in enumerate(bits[:k]):
+                tgt |= ((r >> (k - 1 - i)) & 1) << b
+            np.add.at(out, tgt, m[r, c] * psi[sel])
+    return out
+
+
+@pytest.mark.parametrize("max_k,pad", [(2, 0), (3, 0), (4, 0), (5, 0), (5, 5)])
+@pytest.mark.parametrize("seed", range(4))
+def test_fusion_planner(emu, bt, max_k, pad, seed):
+    """Fused blocks (bt_fuse.hpp) applied in order equal the gate-by-gate
+    oracle run; every block stays within the fusion width."""
+    from bluetangle_jl_amd.circuit import compile_ir
+    rng = np.random.default_rng(7000 + seed)
+    N = 6
+    spec = random_spec(N, 40, rng, kinds=("gate1", "gate2", "ccx") if max_k >= 3 else ("gate1", "gate2"))
+    ir = compile_ir(to_bt(spec, bt), N, False)
+    arr, pool = ir.arrays()
+    cap = 256
+    ks = np.zeros(cap, dtype=np.int32)
+    bits = np.zeros(5 * cap, dtype=np.int32)
+    mats = np.zeros(2 * 1024 * cap, dtype=np.float64)
+    emu.emu_fuse.restype = ctypes.c_int64
+    nb = emu.emu_fuse(N, arr, len(ir.ops), ir.ntop, pool.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(pool.size // 2),
+                      max_k, pad, ks.ctypes.data_as(ctypes.c_void_p), bits.ctypes.data_as(ctypes.c_void_p),
+                      mats.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(cap))
+    assert nb > 0, emu.emu_error()
+    v = rng.normal(size=2 ** N) + 1j * rng.normal(size=2 ** N)
+    psi = v.copy()
+    cm = mats.view(np.complex128)
+    for b in range(nb):
+        k = int(ks[b])
+        assert k <= max(max_k, pad)
+        if pad:
+            assert k == pad
+        D = 1 << k
+        m = cm[1024 * b: 1024 * b + D * D].reshape(D, D, order="F")
+        psi = _apply_block_dense(psi, N, k, list(bits[5 * b: 5 * b + 5]), m)
+    r, _, _, _ = oracle_run(spec, N, None, 0, state=v)
+    assert np.abs(psi - r).max() < 1e-10
+    assert nb < len(spec)  # fusion merged something
