@@ -146,6 +146,61 @@ def serving_leg(bt, args):
                          "MZ p=0.1 per qubit per layer (statevector); complex64")
 
 
+def c5_leg(bt, args):
+    """Large single state: N=33 complex64 (64 GiB), brickwork without noise,
+    gates fused into 5-qubit 32x32 blocks on FP32 MFMA; FP64 norm every 4
+    blocks; error against the same circuit run in complex128 (FP64 mode,
+    128 GiB) on 2^20 sampled amplitudes plus the global fidelity; beside it
+    the register-tile executor on the complex64 state and the unfused
+    one-gate-per-pass kernels on the first layer."""
+    import numpy as np
+    from bluetangle_jl_amd import workloads as W
+    from bluetangle_jl_amd.circuit import Circuit
+    N, layers = args.c5_N, args.c5_layers
+    ops = W.brickwork_ops(N, layers, 2026)
+    c = Circuit(ops, N)
+    amps = float(1 << N)
+    a = bt.zero_state(N, "c64")
+    st, norms = c.run_fused(a, max_k=5, norm_every=4)
+    blocks = st["blocks"]
+    fused_gbs = st["bytes"] / (st["kernel_ms"] * 1e-3) / 1e9
+    tflops = blocks * amps * 256.0 / (st["kernel_ms"] * 1e-3) / 1e12
+    log("c5 fused: %d blocks, %.1f ms, %.0f GB/s" % (blocks, st["kernel_ms"], fused_gbs))
+    r = bt.zero_state(N, "c128")
+    t0 = time.perf_counter()
+    _, sr = c.run_on(r)
+    ref_s = time.perf_counter() - t0
+    rng = np.random.default_rng(33)
+    idx = rng.integers(0, 1 << N, size=1 << 20, dtype=np.uint64)
+    err = float(np.abs(a.gather(idx) - r.gather(idx)).max())
+    ov = a.inner(r)
+    fid = abs(ov) ** 2 / (a.norm2() * r.norm2())
+    del r
+    a.set_basis(0)
+    t0 = time.perf_counter()
+    _, stile = c.run_on(a)
+    tile_s = time.perf_counter() - t0
+    per_layer = len(ops) // layers
+    a.set_basis(0)
+    a.sync()
+    t0 = time.perf_counter()
+    for op in ops[:per_layer]:
+        bt.apply(a, op)
+    a.sync()
+    unf_s = time.perf_counter() - t0
+    unf_gbs = per_layer * 2.0 * amps * 8 / unf_s / 1e9
+    del a
+    return dict(workload="N=%d complex64 brickwork, %d layers, %d gates, no noise" % (N, layers, len(ops)),
+                fused_blocks=blocks, fused_s=round(st["kernel_ms"] / 1e3, 3), fused_gbs=round(fused_gbs, 1),
+                fused_frac=round(fused_gbs / HBM_PEAK_GBS, 4), mfma_tflops=round(tflops, 1),
+                mfma_frac=round(tflops / 157.3, 4), norm_checks=len(norms),
+                max_norm_dev=round(max(abs(x - 1.0) for x in norms), 9) if norms else None,
+                max_amp_err_vs_fp64=err, fidelity_vs_fp64=fid, fp64_reference_s=round(ref_s, 3),
+                tile_executor_s=round(tile_s, 3), tile_executor_passes=int(stile["passes"]),
+                unfused_layer_s=round(unf_s, 3), unfused_gbs=round(unf_gbs, 1),
+                unfused_s_extrapolated=round(unf_s * layers, 2))
+
+
 def cpu_leg(args):
     """The oracle (FP64 CPU restatement of the reference, fast in-place mode,
     OpenMP) on the same circuit.  Bounded sample: the first L layers of one
@@ -190,7 +245,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=24.0)
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: cpu,plain,gate,serve")
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: cpu,plain,gate,serve,c5")
+    ap.add_argument("--c5", action="store_true", help="also run the N=33 fused-block leg")
+    ap.add_argument("--c5-N", type=int, default=33)
+    ap.add_argument("--c5-layers", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (the value is one GPU's "
@@ -243,6 +301,9 @@ def main():
     if "serve" not in skip:
         out["serving"] = serving_leg(bt, args)
         log("serving", out["serving"])
+    if args.c5 and "c5" not in skip:
+        out["large_state"] = c5_leg(bt, args)
+        log("c5", out["large_state"])
     if "cpu" not in skip:
         out["cpu_baseline"] = cpu_leg(args)
         log("cpu", out["cpu_baseline"])

@@ -112,6 +112,12 @@ class DeviceState:
         L.check(L.lib().bt_state_set_basis(self.handle, int(index)))
         return self
 
+    def inner(self, other):
+        """<self|other> in FP64 (either precision on either side)."""
+        re, im = ctypes.c_double(), ctypes.c_double()
+        L.check(L.lib().bt_inner(self.handle, other.handle, ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
+
     def sync(self):
         """Wait for the work queued on this state's stream."""
         L.check(L.lib().bt_state_sync(self.handle))

@@ -569,6 +569,18 @@ int bt_norm2(bt_state s, double* out) {
   return launch_norm2(s, out);
 }
 
+int bt_inner(bt_state a, bt_state b, double* re, double* im) {
+  int rc = check_state(a);
+  if (rc) return rc;
+  rc = check_state(b);
+  if (rc) return rc;
+  if (a->kind != b->kind || a->n != b->n) return fail(BT_ERR_ARG, "bt_inner: states of different shapes");
+  if (!re || !im) return fail(BT_ERR_ARG, "null output");
+  // b's queued work must be done before a's stream reads it
+  BT_HIP(hipStreamSynchronize(b->stream));
+  return launch_inner(a, b, re, im);
+}
+
 int bt_trace_purity(bt_state s, double* trace, double* purity) {
   int rc = check_state(s);
   if (rc) return rc;

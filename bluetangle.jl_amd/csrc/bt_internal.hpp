@@ -138,6 +138,7 @@ int launch_apply(bt_state_s* s, int nbits, int k, const double* mat_colmajor, co
 int launch_reduce(bt_state_s* s, int k, const int* bits);
 void unpack_reduced(int k, const double* packed, double* rho_colmajor);
 int launch_norm2(bt_state_s* s, double* out);
+int launch_inner(bt_state_s* a, bt_state_s* b, double* re, double* im);  // <a|b>, FP64
 int launch_trace(bt_state_s* s, double* out);
 int launch_probabilities(bt_state_s* s, double* d_out);
 int launch_scale(bt_state_s* s, double f);

@@ -126,12 +126,35 @@ __global__ void __launch_bounds__(256) k_reduce(ReduceArgs<R, K> a) {
   block_reduce_store<M>(acc, a.part + (size_t)blockIdx.x * M);
 }
 
-__global__ void k_finalize(const double* part, int nblocks, int M, double* out) {
-  for (int j = threadIdx.x; j < M; j += blockDim.x) {
-    double t = 0.0;
-    for (int b = 0; b < nblocks; b++) t += part[(size_t)b * M + j];
-    out[j] = t;
+// out[j] = sum over blocks of part[b * M + j], one workgroup per j: each
+// thread sums a fixed stride of blocks, then a fixed tree (bit-reproducible).
+__global__ void __launch_bounds__(256) k_finalize(const double* part, int nblocks, int M, double* out) {
+  __shared__ double sh[256];
+  const int j = blockIdx.x;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) t += part[(size_t)b * M + j];
+  sh[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[j] = sh[0];
+}
+
+// <a|b> = sum conj(a_i) b_i in FP64 (either precision on either side):
+// per-workgroup partials (re, im)
+template <typename RA, typename RB>
+__global__ void __launch_bounds__(256) k_inner(const C<RA>* a, const C<RB>* b, uint64_t n, double* part) {
+  double acc[2] = {0.0, 0.0};
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const C<RA> x = a[i];
+    const C<RB> y = b[i];
+    acc[0] = fma((double)x.x, (double)y.x, fma((double)x.y, (double)y.y, acc[0]));
+    acc[1] = fma((double)x.x, (double)y.y, fma(-(double)x.y, (double)y.x, acc[1]));
+  }
+  block_reduce_store<2>(acc, part + (size_t)blockIdx.x * 2);
 }
 
 // sum over i of Re(s[i * step]) for i < count (trace of vec(rho): step = D+1)
@@ -278,10 +301,36 @@ static int reduce_t(bt_state_s* s, const Placement& p) {
   const unsigned blocks = grid_for(a.ngroups, 256, kRedBlocks);
   hipLaunchKernelGGL((k_reduce<R, K>), dim3(blocks), dim3(256), 0, s->stream, a);
   BT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s->stream, (const double*)s->d_part, (int)blocks, M, s->d_out);
+  hipLaunchKernelGGL(k_finalize, dim3(M), dim3(256), 0, s->stream, (const double*)s->d_part, (int)blocks, M, s->d_out);
   BT_HIP(hipGetLastError());
   BT_HIP(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * M, hipMemcpyDeviceToHost, s->stream));
   BT_HIP(hipStreamSynchronize(s->stream));
+  return BT_OK;
+}
+
+int launch_inner(bt_state_s* a, bt_state_s* b, double* re, double* im) {
+  const uint64_t n = a->len;
+  const unsigned blocks = grid_for(n, 256, kRedBlocks);
+  const bool fa = a->prec == BT_C64, fb = b->prec == BT_C64;
+  if (fa && fb)
+    hipLaunchKernelGGL((k_inner<float, float>), dim3(blocks), dim3(256), 0, a->stream, (const C<float>*)a->d,
+                       (const C<float>*)b->d, n, a->d_part);
+  else if (fa)
+    hipLaunchKernelGGL((k_inner<float, double>), dim3(blocks), dim3(256), 0, a->stream, (const C<float>*)a->d,
+                       (const C<double>*)b->d, n, a->d_part);
+  else if (fb)
+    hipLaunchKernelGGL((k_inner<double, float>), dim3(blocks), dim3(256), 0, a->stream, (const C<double>*)a->d,
+                       (const C<float>*)b->d, n, a->d_part);
+  else
+    hipLaunchKernelGGL((k_inner<double, double>), dim3(blocks), dim3(256), 0, a->stream, (const C<double>*)a->d,
+                       (const C<double>*)b->d, n, a->d_part);
+  BT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(2), dim3(256), 0, a->stream, (const double*)a->d_part, (int)blocks, 2, a->d_out);
+  BT_HIP(hipGetLastError());
+  BT_HIP(hipMemcpyAsync(a->h_out, a->d_out, sizeof(double) * 2, hipMemcpyDeviceToHost, a->stream));
+  BT_HIP(hipStreamSynchronize(a->stream));
+  *re = a->h_out[0];
+  *im = a->h_out[1];
   return BT_OK;
 }
 
@@ -332,7 +381,7 @@ int launch_trace(bt_state_s* s, double* out) {
     hipLaunchKernelGGL(k_strided_re_sum<double>, dim3(blocks), dim3(256), 0, s->stream, (const C<double>*)s->d, D,
                        D + 1, s->d_part);
   BT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s->stream, (const double*)s->d_part, (int)blocks, 1, s->d_out);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s->stream, (const double*)s->d_part, (int)blocks, 1, s->d_out);
   BT_HIP(hipGetLastError());
   BT_HIP(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
   BT_HIP(hipStreamSynchronize(s->stream));

@@ -112,6 +112,10 @@ int bt_reset(bt_state s, int bit, double u, int* outcome);
 /* |psi_i|^2 or diag(rho) in FP64 (sample_exact, ops.jl:98-132) */
 int bt_probabilities(bt_state s, double* out);
 int bt_norm2(bt_state s, double* out);
+/* <a|b> = sum conj(a_i) b_i in FP64, either precision on either side (same
+ * shape): fidelity |<a|b>|^2 / (<a|a> <b|b>) of a complex64 result against
+ * its complex128 counterpart. */
+int bt_inner(bt_state a, bt_state b, double* re, double* im);
 /* DM: FP64 trace and purity sum |rho_ij|^2 */
 int bt_trace_purity(bt_state s, double* trace, double* purity);
 /* Amplitudes (or rho entries) at n indices, complex128 interleaved out. */

@@ -87,3 +87,15 @@ def test_fused_rejects_noisy_circuits(bt):
     c = Circuit([bt.Op("H", 1), bt.Op("MZ", 1)], 12)
     with pytest.raises(Exception):
         c.run_fused(bt.zero_state(12, "c64"))
+
+
+def test_inner_across_precisions(bt):
+    rng = np.random.default_rng(3)
+    N = 14
+    v = rng.normal(size=2 ** N) + 1j * rng.normal(size=2 ** N)
+    w = rng.normal(size=2 ** N) + 1j * rng.normal(size=2 ** N)
+    a64, a128 = bt.from_numpy(v, "c64"), bt.from_numpy(v, "c128")
+    b128 = bt.from_numpy(w, "c128")
+    assert abs(a128.inner(b128) - np.vdot(v, w)) < 1e-9 * np.abs(v).sum()
+    assert abs(a64.inner(a128) - np.vdot(v, v)) < 1e-5 * np.vdot(v, v).real
+    assert abs(a128.inner(a128).real - a128.norm2()) < 1e-9
