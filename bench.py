@@ -50,6 +50,10 @@ def c3_leg(bt, args):
     from bluetangle_jl_amd.circuit import Circuit
     cfg = W.config_c3(args.N, args.layers)
     c = Circuit(cfg["ops"], cfg["N"], cfg["noise"])
+    tuning = None
+    if args.autotune > 0:  # untimed: pick tile bits / coalescing / tiles per workgroup on a sample
+        tuning = c.autotune("c64", sample=args.autotune)
+        log("autotune", tuning)
     stats = []
     for w in range(args.warmup):
         c.run_batch(args.ntraj, seed=1000 + w, traj0=0, prec="c64", mode=args.mode)
@@ -60,7 +64,7 @@ def c3_leg(bt, args):
         stats.append(st)
         log("step %d: %.3f s device, %d launches" % (k, st["kernel_ms"] / 1e3, st["launches"]))
     dt = time.perf_counter() - t0
-    return c, cfg, stats, dt
+    return c, cfg, stats, dt, tuning
 
 
 def plain_loop_leg(bt, c, args):
@@ -70,6 +74,20 @@ def plain_loop_leg(bt, c, args):
     t0 = time.perf_counter()
     for t in range(n):
         c.run_on(st.set_basis(0), seed=7, traj=t)
+    return n / (time.perf_counter() - t0)
+
+
+def per_op_loop_leg(bt, cfg, args):
+    """The plain form: apply(state, op; noise) op by op (one or two kernel
+    launches per op, host draws), one trajectory at a time."""
+    n = args.per_op_traj
+    st = bt.zero_state(cfg["N"], "c64")
+    t0 = time.perf_counter()
+    for t in range(n):
+        bt.seed(7, t)
+        st.set_basis(0)
+        bt.apply(st, cfg["ops"], noise=cfg["noise"], track_measurements=True)
+    st.sync()
     return n / (time.perf_counter() - t0)
 
 
@@ -239,7 +257,9 @@ def main():
     ap.add_argument("--N", type=int, default=22)
     ap.add_argument("--layers", type=int, default=32)
     ap.add_argument("--mode", default="rounds", choices=["rounds", "persistent"])
+    ap.add_argument("--autotune", type=int, default=256, help="trajectories per autotuner candidate (0: off)")
     ap.add_argument("--plain-traj", type=int, default=64)
+    ap.add_argument("--per-op-traj", type=int, default=4)
     ap.add_argument("--gate-N", type=int, default=30)
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=24.0)
@@ -258,7 +278,7 @@ def main():
     import __graft_entry__ as G
     bt = G.load_package()
     bt.init()
-    c, cfg, stats, dt = c3_leg(bt, args)
+    c, cfg, stats, dt, tuning = c3_leg(bt, args)
     value = args.ntraj * args.steps / dt
     kms = sum(s["kernel_ms"] for s in stats)
     nbytes = sum(s["bytes"] for s in stats)
@@ -283,7 +303,8 @@ def main():
                                % (args.ntraj, args.N, args.layers),
                    "ntraj": args.ntraj, "N": args.N, "layers": args.layers, "executor": args.mode,
                    "passes_per_trajectory": round(passes, 3),
-                   "boundary_draws": int(sum(s["boundary_draws"] for s in stats))},
+                   "boundary_draws": int(sum(s["boundary_draws"] for s in stats)),
+                   "tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": "k_round<float>" if args.mode == "rounds" else "k_batch<float>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -293,8 +314,14 @@ def main():
     if "plain" not in skip:
         pl = plain_loop_leg(bt, c, args)
         out["plain_loop"] = {"value": round(pl, 3), "unit": "trajectories/s", "sample": args.plain_traj,
-                             "batched_speedup": round(value / pl, 2)}
+                             "batched_speedup": round(value / pl, 2),
+                             "executor": "bt_run_circuit per trajectory (fused tile passes, one launch per pass)"}
         log("plain loop %.2f traj/s" % pl)
+        po = per_op_loop_leg(bt, cfg, args)
+        out["per_op_loop"] = {"value": round(po, 4), "unit": "trajectories/s", "sample": args.per_op_traj,
+                              "batched_speedup": round(value / po, 1),
+                              "executor": "apply(state, op; noise) per op, one trajectory at a time"}
+        log("per-op loop %.3f traj/s" % po)
     if "gate" not in skip:
         out["gate_layer"] = gate_layer_leg(bt, args)
         log("gate layer", out["gate_layer"])

@@ -29,7 +29,8 @@ SYMBOLS = [
     "bt_apply_matrix", "bt_reduced_rho", "bt_apply_kraus_traj", "bt_apply_kraus_dm", "bt_measure", "bt_reset",
     "bt_probabilities", "bt_norm2", "bt_trace_purity", "bt_gather",
     "bt_circuit_create", "bt_circuit_destroy", "bt_circuit_max_draws", "bt_run_circuit", "bt_run_circuit_dm",
-    "bt_run_batch", "bt_run_batch_persistent", "bt_set_tile_bits",
+    "bt_run_batch", "bt_run_batch_persistent", "bt_set_tile_bits", "bt_set_tuning", "bt_get_tuning",
+    "bt_autotune_batch",
     "bt_server_create", "bt_server_submit", "bt_server_poll", "bt_server_fetch", "bt_server_stats",
     "bt_server_destroy",
 ]
@@ -55,6 +56,14 @@ class RunStats(ctypes.Structure):
     _fields_ = [("passes", ctypes.c_int64), ("reduce_passes", ctypes.c_int64), ("bytes", ctypes.c_double),
                 ("boundary_draws", ctypes.c_int64), ("draws", ctypes.c_int64), ("kernel_ms", ctypes.c_double),
                 ("launches", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("tile_bits", ctypes.c_int32), ("coalesce_bits", ctypes.c_int32), ("tiles_per_block", ctypes.c_int32),
+                ("ms_per_traj", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -123,6 +132,10 @@ def lib():
                                                        ctypes.c_int64, ip, ctypes.c_int64, i64p, dp,
                                                        ctypes.POINTER(RunStats)]),
             "bt_set_tile_bits": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+            "bt_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "bt_get_tuning": (ctypes.c_int, [ctypes.c_int, ip, ip, ip]),
+            "bt_autotune_batch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64,
+                                                 ctypes.POINTER(Tuning)]),
             "bt_server_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(vp)]),
             "bt_server_submit": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                                 ctypes.c_uint64, i64p]),

@@ -230,6 +230,15 @@ class Circuit:
         nn = min(cap, nb.value // norm_every) if norm_every > 0 else 0
         return d, norms[:nn].tolist()
 
+    def autotune(self, prec="c64", sample=64, seed=12345):
+        """Pick the batched executor's tile bits, coalescing reserve and tiles
+        per workgroup by timing `sample` trajectories of this circuit per
+        candidate; the winner stays in effect (process-wide)."""
+        from .core import PRECISIONS
+        t = L.Tuning()
+        L.check(L.lib().bt_autotune_batch(self.handle, PRECISIONS[prec], sample, seed, ctypes.byref(t)))
+        return t.as_dict()
+
     def run_dm(self, rho: DensityMatrix):
         st = L.RunStats()
         L.check(L.lib().bt_run_circuit_dm(rho.handle, self.handle, ctypes.byref(st)))

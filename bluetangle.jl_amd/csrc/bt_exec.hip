@@ -195,9 +195,15 @@ static int upload_circuit(bt_circuit_s* c) {
   return BT_OK;
 }
 
-// Tile width for a state of nbits: the largest that fits the LDS budget of a
-// 512-thread workgroup, or an override from the autotuner.
+// Tile width for a state of nbits: the register tile of a 256-thread
+// workgroup (2^(8 + register bits) amplitudes), or an override from the
+// autotuner; likewise the coalescing reserve and the rounds executor's
+// tiles per workgroup.
 static int g_tile_override[2] = {0, 0};
+static int g_coalesce_override[2] = {0, 0};
+static int g_tpb_override = 0;
+
+int tiles_per_block() { return g_tpb_override ? g_tpb_override : 8; }
 
 int default_tile_bits(int prec) {
   return prec == BT_C64 ? TileCfg<float>::LOGNT + TileCfg<float>::RB : TileCfg<double>::LOGNT + TileCfg<double>::RB;
@@ -205,7 +211,10 @@ int default_tile_bits(int prec) {
 
 // Low global bits every tile keeps (runs of 2^c consecutive amplitudes per
 // wavefront access): 128 B for either precision.
-int coalesce_bits(int prec) { return prec == BT_C64 ? 4 : 3; }
+int coalesce_bits(int prec) {
+  const int p = prec == BT_C64 ? 0 : 1;
+  return g_coalesce_override[p] ? g_coalesce_override[p] : (prec == BT_C64 ? 4 : 3);
+}
 
 int tile_bits_for(int prec, int nbits) {
   int T = g_tile_override[prec] ? g_tile_override[prec] : default_tile_bits(prec);
@@ -569,6 +578,79 @@ int bt_set_tile_bits(int prec, int T) {
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
   if (T != 0 && (T < 2 || T > default_tile_bits(prec))) return fail(BT_ERR_ARG, "tile bits out of range");
   g_tile_override[prec] = T;
+  return BT_OK;
+}
+
+int bt_set_tuning(int prec, int tile_bits, int coalesce, int tiles_per_blk) {
+  if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
+  if (tile_bits != 0 && (tile_bits < 6 || tile_bits > default_tile_bits(prec)))
+    return fail(BT_ERR_ARG, "tile bits out of range");
+  if (coalesce < 0 || coalesce > 6) return fail(BT_ERR_ARG, "coalescing bits out of range");
+  if (tiles_per_blk < 0 || tiles_per_blk > 64) return fail(BT_ERR_ARG, "tiles per workgroup out of range");
+  g_tile_override[prec] = tile_bits;
+  g_coalesce_override[prec] = coalesce;
+  g_tpb_override = tiles_per_blk;
+  return BT_OK;
+}
+
+int bt_get_tuning(int prec, int* tile_bits, int* coalesce, int* tiles_per_blk) {
+  if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
+  if (tile_bits) *tile_bits = g_tile_override[prec] ? g_tile_override[prec] : default_tile_bits(prec);
+  if (coalesce) *coalesce = coalesce_bits(prec);
+  if (tiles_per_blk) *tiles_per_blk = tiles_per_block();
+  return BT_OK;
+}
+
+// Autotuner of the batched executor: time each candidate (tile bits,
+// coalescing reserve, tiles per workgroup) on `sample` trajectories of the
+// circuit (device time of the round passes) and keep the fastest.
+int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_tuning* best) {
+  if (!c || !best) return fail(BT_ERR_ARG, "null circuit or result");
+  if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
+  if (sample < 1) return fail(BT_ERR_ARG, "sample must be positive");
+  const int p = prec == BT_C64 ? 0 : 1;
+  const int saved[3] = {g_tile_override[p], g_coalesce_override[p], g_tpb_override};
+  const int Tmax = default_tile_bits(prec);
+  const int nb = c->h.n;
+  std::vector<int> Ts;
+  for (int T = Tmax; T >= Tmax - 1; T--)
+    if (T <= nb) Ts.push_back(T);
+  if (Ts.empty()) Ts.push_back(std::min(Tmax, nb));
+  const int outcap = (int)std::max<size_t>(16, 2 * c->h.ops.size());
+  std::vector<int> outs((size_t)sample * outcap);
+  std::vector<int64_t> nouts(sample);
+  double best_ms = -1;
+  bt_tuning b{};
+  int rc = BT_OK;
+  for (int T : Ts)
+    for (int cb : {3, 4, 5})
+      for (int tpb : {4, 8, 16}) {
+        if (cb >= T) continue;
+        g_tile_override[p] = T;
+        g_coalesce_override[p] = cb;
+        g_tpb_override = tpb;
+        bt_run_stats st;
+        rc = bt_run_batch(c, prec, seed, 0, sample, outs.data(), outcap, nouts.data(), nullptr, &st);
+        if (rc) break;
+        const double ms = st.kernel_ms / (double)sample;
+        if (best_ms < 0 || ms < best_ms) {
+          best_ms = ms;
+          b.tile_bits = T;
+          b.coalesce_bits = cb;
+          b.tiles_per_block = tpb;
+          b.ms_per_traj = ms;
+        }
+      }
+  if (rc || best_ms < 0) {
+    g_tile_override[p] = saved[0];
+    g_coalesce_override[p] = saved[1];
+    g_tpb_override = saved[2];
+    return rc ? rc : fail(BT_ERR_STATE, "autotune: no candidate ran");
+  }
+  g_tile_override[p] = b.tile_bits;
+  g_coalesce_override[p] = b.coalesce_bits;
+  g_tpb_override = b.tiles_per_block;
+  *best = b;
   return BT_OK;
 }
 

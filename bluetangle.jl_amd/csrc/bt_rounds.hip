@@ -34,6 +34,7 @@ namespace bt {
 
 int tile_bits_for(int prec, int nbits);
 int coalesce_bits(int prec);
+int tiles_per_block();
 
 // ---------------------------------------------------------------------------
 // device side
@@ -49,7 +50,7 @@ struct RoundItem {
 
 // table layout: PassGeom, then int32 nmats (padded to 16 B), then nmats matrix entries
 constexpr size_t kGeomBytes = (sizeof(PassGeom) + 15) / 16 * 16;
-constexpr uint32_t kTilesPerBlock = 8;
+
 
 // One workgroup per `tpb` consecutive tiles of one item.  nb_uniform != 0:
 // every item has that many workgroups (the block -> item map is a division),
@@ -333,7 +334,7 @@ class RoundEngine {
     // enough that the largest item still spreads over the chip
     uint32_t max_tiles = 0;
     for (Job* j : act) max_tiles = std::max<uint32_t>(max_tiles, (uint32_t)(1ull << (j->nbits - j->plan.ntile)));
-    const int tpb = (int)std::max<uint32_t>(1, std::min<uint32_t>(kTilesPerBlock, max_tiles / 64));
+    const int tpb = (int)std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)tiles_per_block(), max_tiles / 64));
     size_t blob = 0;
     uint32_t blocks = 0;
     std::vector<RoundItem> items(act.size());

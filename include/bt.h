@@ -198,6 +198,21 @@ int bt_run_fused(bt_state s, bt_circuit c, int max_k, int norm_every, double* no
  * the autotuner.  Process-wide. */
 int bt_set_tile_bits(int prec, int tile_bits);
 
+/* Tuning of the circuit executors (process-wide, per precision): tile bits,
+ * low global bits every tile keeps for coalescing, tiles per workgroup of the
+ * rounds executor; 0 restores a default. */
+typedef struct {
+  int32_t tile_bits;
+  int32_t coalesce_bits;
+  int32_t tiles_per_block;
+  double ms_per_traj;  /* autotune: device time per trajectory of the winner */
+} bt_tuning;
+int bt_set_tuning(int prec, int tile_bits, int coalesce_bits, int tiles_per_block);
+int bt_get_tuning(int prec, int* tile_bits, int* coalesce_bits, int* tiles_per_block);
+/* Autotuner: runs `sample` trajectories of the circuit through bt_run_batch
+ * for each candidate setting, keeps (and returns) the fastest. */
+int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_tuning* best);
+
 /* ---- serving: continuous batching with HBM-budget admission ------------- */
 typedef struct {
   int64_t id;

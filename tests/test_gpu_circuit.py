@@ -158,3 +158,27 @@ def test_run_driver_kats(bt):
         assert run(ops_reset, 3, batched=batched) == [[0]] * 3
         assert run(ops_ifop, 3, batched=batched) == [[1]] * 3
         assert run(ops_mid, 3, batched=batched) == [[1]] * 3
+
+
+def test_autotuner_keeps_results(bt):
+    """The autotuner picks a valid tuning; trajectories run under it still
+    match the oracle (tuning changes the pass plan, not the results)."""
+    from bluetangle_jl_amd import _lib
+    from bluetangle_jl_amd.circuit import Circuit
+    N = 12
+    spec = brickwork_spec(N, 5, 3, ad_gamma=5e-3, mz_prob=0.2)
+    nm_args = ("depolarizing", 1e-3, "depolarizing", 1e-2)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
+    try:
+        t = c.autotune("c64", sample=8)
+        assert 6 <= t["tile_bits"] <= 13 and t["coalesce_bits"] in (3, 4, 5) and t["tiles_per_block"] in (4, 8, 16)
+        assert t["ms_per_traj"] > 0
+        outs, states, _ = c.run_batch(12, seed=5, prec="c64", keep_states=True)
+        for tr in range(12):
+            r, routs, bnd = oracle_traj(N, spec, nm_args, 5, tr)
+            if bnd:
+                continue
+            assert outs[tr] == routs
+            assert np.abs(states[tr] - r).max() < 1e-5
+    finally:
+        _lib.check(_lib.lib().bt_set_tuning(0, 0, 0, 0))
