@@ -39,6 +39,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "noisy trajectories/s + achieved HBM GB/s per gate layer; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+# k_round's measured HBM traffic / algorithmic bytes on this circuit (rocprofv3
+# FETCH_SIZE + WRITE_SIZE, copy-calibrated); re-measured whenever the kernel changes
+TRAFFIC_RATIO = 1.178
 
 
 def log(*a):
@@ -51,7 +54,12 @@ def c3_leg(bt, args):
     cfg = W.config_c3(args.N, args.layers)
     c = Circuit(cfg["ops"], cfg["N"], cfg["noise"])
     tuning = None
-    if args.autotune > 0:  # untimed: pick tile bits / coalescing / tiles per workgroup on a sample
+    if args.tuning:  # explicit tile bits, coalescing bits, tiles per workgroup
+        from bluetangle_jl_amd import _lib as L
+        T, cb, tpb = (int(x) for x in args.tuning.split(","))
+        L.check(L.lib().bt_set_tuning(0, T, cb, tpb))
+        tuning = {"tile_bits": T, "coalesce_bits": cb, "tiles_per_block": tpb, "ms_per_traj": None}
+    elif args.autotune > 0:  # untimed: pick tile bits / coalescing / tiles per workgroup on a sample
         tuning = c.autotune("c64", sample=args.autotune)
         log("autotune", tuning)
     stats = []
@@ -258,6 +266,7 @@ def main():
     ap.add_argument("--layers", type=int, default=32)
     ap.add_argument("--mode", default="rounds", choices=["rounds", "persistent"])
     ap.add_argument("--autotune", type=int, default=256, help="trajectories per autotuner candidate (0: off)")
+    ap.add_argument("--tuning", default="", help="T,C,B: fixed tile bits, coalescing bits, tiles per workgroup")
     ap.add_argument("--plain-traj", type=int, default=64)
     ap.add_argument("--per-op-traj", type=int, default=4)
     ap.add_argument("--gate-N", type=int, default=30)
@@ -307,7 +316,11 @@ def main():
                    "tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": "k_round<float>" if args.mode == "rounds" else "k_batch<float>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(nbytes / max(1, launches) * TRAFFIC_RATIO, 1),
+                     "traffic_source": "HBM read+write per launch = algorithmic x %.3f, the FETCH_SIZE/WRITE_SIZE "
+                                       "ratio measured for this kernel and circuit (profiles/k_round_traffic_pmc.txt)"
+                                       % TRAFFIC_RATIO,
                      "launches": int(launches), "avg_launch_ms": round(kms / max(1, launches), 4),
                      "bytes_per_launch": round(nbytes / max(1, launches), 1)},
     }
