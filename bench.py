@@ -19,10 +19,15 @@ Beside it, on the same JSON line:
                 circuit;
   plain_loop    the same circuit one trajectory per call (bt_run_circuit: one
                 launch per pass, one circuit per launch);
+  per_op_loop   the plain form: apply(state, op; noise) op by op;
   gate_layer    single-gate layers on an N=30 complex64 state (per-op
                 kernels), achieved GB/s against 8 TB/s;
   serving       Poisson arrivals of mixed circuits through the continuous-
-                batching server: p50/p99 latency and throughput.
+                batching server: p50/p99 latency and throughput;
+  large_state   N=33 complex64 brickwork: 5-qubit blocks on FP32 MFMA vs the
+                register-tile executor vs unfused kernels, error and fidelity
+                against the complex128 run.
+Before the timed steps the autotuner picks the tiling on a sample (untimed).
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--ntraj 4096]
 """
@@ -275,7 +280,7 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--skip", default="", help="comma list of legs to skip: cpu,plain,gate,serve,c5")
-    ap.add_argument("--c5", action="store_true", help="also run the N=33 fused-block leg")
+    ap.add_argument("--no-c5", dest="c5", action="store_false", help="skip the N=33 large-state leg")
     ap.add_argument("--c5-N", type=int, default=33)
     ap.add_argument("--c5-layers", type=int, default=40)
     args = ap.parse_args()

@@ -231,12 +231,22 @@ class RoundEngine {
   explicit RoundEngine(int prec) : prec_(prec), pool_(pool_threads()) {}
   ~RoundEngine() { release(); }
 
-  int init() {
+  // shared != nullptr: launch on that stream (not owned), e.g. two engines
+  // whose rounds alternate on one stream so the GPU runs one cohort while
+  // the host plans the other
+  int init(hipStream_t shared = nullptr) {
     int rc = ensure_device();
     if (rc) return rc;
-    BT_HIP(hipStreamCreate(&stream_));
+    if (shared) {
+      stream_ = shared;
+      own_stream_ = false;
+    } else {
+      BT_HIP(hipStreamCreate(&stream_));
+      own_stream_ = true;
+    }
     BT_HIP(hipEventCreate(&ev_a_));
     BT_HIP(hipEventCreate(&ev_b_));
+    BT_HIP(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming));
     return BT_OK;
   }
 
@@ -251,8 +261,9 @@ class RoundEngine {
     if (h_out_) (void)hipHostFree(h_out_);
     if (ev_a_) (void)hipEventDestroy(ev_a_);
     if (ev_b_) (void)hipEventDestroy(ev_b_);
-    if (stream_) (void)hipStreamDestroy(stream_);
-    ev_a_ = ev_b_ = nullptr;
+    if (ev_done_) (void)hipEventDestroy(ev_done_);
+    if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
+    ev_a_ = ev_b_ = ev_done_ = nullptr;
     h_blob_ = d_blob_ = nullptr;
     h_items_ = nullptr;
     d_items_ = nullptr;
@@ -286,6 +297,14 @@ class RoundEngine {
 
   // One round over `jobs` (all unfinished).  Returns BT_OK or an error.
   int round(std::vector<Job*>& jobs) {
+    int rc = launch_round(jobs);
+    if (rc) return rc;
+    return finish_round();
+  }
+
+  // Plan and enqueue the next pass of every job; finish_round() waits for it
+  // and hands the reductions back.  Returns BT_OK or an error.
+  int launch_round(std::vector<Job*>& jobs) {
     // 1. advance every interpreter to its next pass and build its table
     pool_.parallel_for((int)jobs.size(), [&](int i) {
       Job* j = jobs[i];
@@ -326,15 +345,17 @@ class RoundEngine {
       j->table.resize(kGeomBytes + 16 + elem() * nm);
     });
     // 2. pack
-    std::vector<Job*> act;
+    act_.clear();
     for (Job* j : jobs)
-      if (j->has_plan) act.push_back(j);
+      if (j->has_plan) act_.push_back(j);
+    std::vector<Job*>& act = act_;
+    pending_ = false;
     if (act.empty()) return BT_OK;
     // tiles per workgroup: enough to amortise the workgroup's set-up, few
     // enough that the largest item still spreads over the chip
     uint32_t max_tiles = 0;
     for (Job* j : act) max_tiles = std::max<uint32_t>(max_tiles, (uint32_t)(1ull << (j->nbits - j->plan.ntile)));
-    const int tpb = (int)std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)tiles_per_block(), max_tiles / 64));
+    const int tpb = (int)std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)tiles_per_block(), max_tiles / 32));
     size_t blob = 0;
     uint32_t blocks = 0;
     std::vector<RoundItem> items(act.size());
@@ -382,14 +403,22 @@ class RoundEngine {
                        (uint64_t)blocks, d_out_);
     BT_HIP(hipGetLastError());
     BT_HIP(hipMemcpyAsync(h_out_, d_out_, sizeof(double) * kAcc * items.size(), hipMemcpyDeviceToHost, stream_));
-    BT_HIP(hipStreamSynchronize(stream_));
+    BT_HIP(hipEventRecord(ev_done_, stream_));
+    pending_ = true;
+    launched_blocks_ += blocks;
+    return BT_OK;
+  }
+
+  int finish_round() {
+    if (!pending_) return BT_OK;
+    pending_ = false;
+    BT_HIP(hipEventSynchronize(ev_done_));
     float ms = 0;
     if (hipEventElapsedTime(&ms, ev_a_, ev_b_) == hipSuccess) pass_ms_ += ms;
     rounds_++;
-    launched_blocks_ += blocks;
     // 4. hand the results back
-    for (size_t i = 0; i < act.size(); i++) {
-      Job* j = act[i];
+    for (size_t i = 0; i < act_.size(); i++) {
+      Job* j = act_[i];
       to_result(j->plan, j->red_swap, h_out_ + kAcc * i, &j->res);
       if (j->plan.write) j->fresh = 0;
     }
@@ -441,6 +470,10 @@ class RoundEngine {
   int prec_;
   Pool pool_;
   hipStream_t stream_ = nullptr;
+  bool own_stream_ = true;
+  hipEvent_t ev_done_ = nullptr;
+  std::vector<Job*> act_;
+  bool pending_ = false;
   void* h_blob_ = nullptr;
   void* d_blob_ = nullptr;
   size_t cap_blob_ = 0;
@@ -485,12 +518,18 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
   if (!c) return fail(BT_ERR_ARG, "null circuit");
   if (ntraj <= 0 || !outcomes || !nouts || outcap < 0) return fail(BT_ERR_ARG, "bad batch arguments");
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
-  RoundEngine eng(prec);
-  int rc = eng.init();
+  // two cohorts whose rounds alternate on one stream: while the GPU runs one
+  // cohort's pass, the host plans the other's next pass
+  RoundEngine eng0(prec), eng1(prec);
+  int rc = eng0.init();
   if (rc) return rc;
+  rc = eng1.init(eng0.stream());
+  if (rc) return rc;
+  RoundEngine* E[2] = {&eng0, &eng1};
+  hipStream_t stream = eng0.stream();
   auto circ = std::shared_ptr<const btc::CircuitHost>(&c->h, [](const btc::CircuitHost*) {});
   const int nbits = c->h.n;
-  const size_t bytes = (size_t)(1ull << nbits) * eng.elem();
+  const size_t bytes = (size_t)(1ull << nbits) * eng0.elem();
   size_t freeb = 0, totalb = 0;
   BT_HIP(hipMemGetInfo(&freeb, &totalb));
   // resident states: all of them when they fit in 3/4 of free HBM
@@ -519,18 +558,15 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
   std::vector<std::unique_ptr<Job>> jobs(ntraj);
   std::vector<int64_t> free_slots;
   for (int64_t s = nslots - 1; s >= 0; s--) free_slots.push_back(s);
-  std::vector<Job*> active;
+  std::vector<Job*> active[2];
+  const int64_t cap[2] = {(nslots + 1) / 2, nslots / 2};
   std::vector<int64_t> slot_of(ntraj, -1);
   int64_t next = 0, done = 0;
-  hipEvent_t e0, e1;
-  BT_HIP(hipEventCreate(&e0));
-  BT_HIP(hipEventCreate(&e1));
-  (void)hipEventRecord(e0, eng.stream());
   bt_run_stats acc;
   memset(&acc, 0, sizeof acc);
   int first_err = 0;
-  while (done < ntraj) {
-    while (next < ntraj && !free_slots.empty()) {
+  auto admit = [&](int ch) {
+    while (next < ntraj && !free_slots.empty() && (int64_t)active[ch].size() < cap[ch]) {
       const int64_t s = free_slots.back();
       free_slots.pop_back();
       jobs[next].reset(new Job());
@@ -544,19 +580,19 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
       j->state = (char*)slot_mem[0] + bytes * s;
       j->bytes = bytes;
       slot_of[next] = s;
-      eng.start_job(j);
-      active.push_back(j);
+      E[ch]->start_job(j);
+      active[ch].push_back(j);
       next++;
     }
-    rc = eng.round(active);
-    if (rc) break;
+  };
+  auto retire = [&](int ch) -> int {
     std::vector<Job*> still;
-    for (Job* j : active) {
+    for (Job* j : active[ch]) {
       if (!j->done) {
         still.push_back(j);
         continue;
       }
-      RoundEngine::finish_stats(j, eng.elem());
+      RoundEngine::finish_stats(j, eng0.elem());
       acc.passes += j->passes;
       acc.reduce_passes += j->rpasses;
       acc.boundary_draws += j->boundary;
@@ -568,24 +604,36 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
       for (int64_t i = 0; i < n; i++) outcomes[t * outcap + i] = j->outs[i];
       nouts[t] = j->ip->nout;
       if (final_states) {
-        rc = export_state(j, eng.stream(), final_states + (uint64_t)t * 2 * (1ull << nbits), d_tmp);
-        if (rc) break;
+        const int erc = export_state(j, stream, final_states + (uint64_t)t * 2 * (1ull << nbits), d_tmp);
+        if (erc) return erc;
       }
       free_slots.push_back(slot_of[t]);
       jobs[t].reset();
       done++;
     }
-    if (rc) break;
-    active.swap(still);
+    active[ch].swap(still);
+    return BT_OK;
+  };
+  for (int ch = 0; ch < 2 && rc == BT_OK; ch++) {
+    admit(ch);
+    rc = E[ch]->launch_round(active[ch]);
   }
-  (void)hipEventRecord(e1, eng.stream());
-  (void)hipStreamSynchronize(eng.stream());
+  while (rc == BT_OK && done < ntraj) {
+    for (int ch = 0; ch < 2 && rc == BT_OK; ch++) {
+      rc = E[ch]->finish_round();
+      if (rc == BT_OK) rc = retire(ch);
+      if (rc == BT_OK) {
+        admit(ch);
+        rc = E[ch]->launch_round(active[ch]);
+      }
+    }
+  }
+  for (int ch = 0; ch < 2; ch++) (void)E[ch]->finish_round();
+  (void)hipStreamSynchronize(stream);
   // device time of the pass kernels (HIP events around each launch)
-  acc.kernel_ms = eng.pass_ms();
-  acc.launches = eng.rounds();
+  acc.kernel_ms = eng0.pass_ms() + eng1.pass_ms();
+  acc.launches = eng0.rounds() + eng1.rounds();
   if (stats) *stats = acc;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   cleanup();
   if (rc) return rc;
   if (first_err) return fail(first_err, "bt_run_batch: a trajectory failed (interpreter error " +
