@@ -70,11 +70,22 @@ struct ROp {
   uint64_t gctrl;      // control bits outside the tile (global positions)
 };
 
+// LDS slots of a layout, precomputed per pass: the tile image is indexed by
+// a swizzled local index and the swizzle is XOR-linear, so the slot of
+// (thread base | register offset) is the XOR of the slots of its bits.
+struct LSlots {
+  uint16_t ts[8];    // slot of thread-index bit i (0 where the thread bit is unused)
+  uint16_t rs[kRB];  // slot of register position j (0 for phantoms)
+  uint16_t pad[3];
+};
+
 struct RPhase {
   int8_t rb[kRB];  // local tile bit held at register position j (-1: phantom)
   int8_t pad[3];
   int32_t op0, nops;
 };
+
+constexpr int kLayIn = kMaxPh, kLayOut = kMaxPh + 1;  // slot tables of the load / store layouts
 
 struct PassGeom {
   int32_t T, nrb, tb, ncomp, write, need_norm, red, red_k, red_swap, nph, nops, nmats;
@@ -86,6 +97,7 @@ struct PassGeom {
   int32_t red_bits[4];       // RED_HIST: global bits; RED_RHO: register positions (last phase), ascending
   int8_t io_in[kRB], io_out[kRB];
   int8_t pad[6];
+  LSlots lay[kMaxPh + 2];    // slot tables: phases, then kLayIn, kLayOut
   double scale;
   RPhase ph[kMaxPh];
   ROp ops[bti::kPMax];
@@ -107,6 +119,29 @@ __host__ __device__ inline int popc32(uint32_t x) {
 // (ascending) and the rest phantom.
 __host__ __device__ inline void set_layout(int8_t* rb, const int* bits, int n) {
   for (int j = 0; j < kRB; j++) rb[j] = j < n ? (int8_t)bits[j] : (int8_t)-1;
+}
+
+// LDS element swizzle of local index l: XOR the low W bits with higher bits
+// so strided layouts spread over the banks (W = 5 for 8-byte, 4 for 16-byte
+// elements).  Linear over XOR.
+template <typename R>
+__host__ __device__ __forceinline__ uint32_t swz(uint32_t l) {
+  constexpr int W = sizeof(C<R>) == 8 ? 5 : 4;
+  return l ^ (((l >> W) ^ (l >> (2 * W))) & ((1u << W) - 1u));
+}
+
+template <typename R>
+__host__ __device__ inline void make_slots(const int8_t* rb, int T, LSlots* s) {
+  uint32_t regm = 0;
+  for (int j = 0; j < kRB; j++) {
+    if (rb[j] >= 0) regm |= 1u << rb[j];
+    s->rs[j] = rb[j] >= 0 ? (uint16_t)swz<R>(1u << rb[j]) : (uint16_t)0;
+  }
+  int i = 0;
+  for (int b = 0; b < T && i < 8; b++)
+    if (!((regm >> b) & 1)) s->ts[i++] = (uint16_t)swz<R>(1u << b);
+  for (; i < 8; i++) s->ts[i] = 0;
+  s->pad[0] = s->pad[1] = s->pad[2] = 0;
 }
 
 template <typename R>
@@ -373,6 +408,9 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
     moff += D2 * D2;
   }
   g->nmats = moff;
+  for (int p = 0; p < nph; p++) make_slots<R>(g->ph[p].rb, T, &g->lay[p]);
+  make_slots<R>(g->io_in, T, &g->lay[kLayIn]);
+  make_slots<R>(g->io_out, T, &g->lay[kLayOut]);
   // 5. reductions
   if (plan.red == bti::RED_HIST) {
     for (int i = 0; i < plan.red_k; i++) g->red_bits[i] = plan.red_bits[i];
@@ -512,13 +550,6 @@ __host__ __device__ __forceinline__ void rho_mask(int mask, const C<R>* v, int n
   }
 }
 
-// LDS element index swizzle: XOR the low W bits with higher bits so strided
-// layouts spread over the banks (W = 5 for 8-byte, 4 for 16-byte elements).
-template <typename R>
-__host__ __device__ __forceinline__ uint32_t swz(uint32_t l) {
-  constexpr int W = sizeof(C<R>) == 8 ? 5 : 4;
-  return l ^ (((l >> W) ^ (l >> (2 * W))) & ((1u << W) - 1u));
-}
 
 __host__ __device__ __forceinline__ uint32_t ins0(uint32_t x, int b) { return ((x >> b) << (b + 1)) | (x & ((1u << b) - 1u)); }
 
@@ -613,40 +644,36 @@ __host__ __device__ __forceinline__ uint32_t uni32(uint32_t x) {
 #endif
 }
 
-// LDS slots of a layout.  The swizzle is linear over XOR, so the slot of
-// (thread base | register offset) is swz(base) ^ XOR of swz(bit) over the
-// register's bits: one VALU XOR per register against workgroup-uniform
-// (scalar) terms.
-template <typename R>
-struct LdsSlots {
-  uint32_t s0;        // swz(thread base), per thread
-  uint32_t sj[kRB];   // swz(1 << register bit j), uniform
-  __host__ __device__ __forceinline__ LdsSlots(const int8_t* lay, int T, uint32_t tid) {
-    s0 = swz<R>(thread_base(lay, T, tid));
+// Slot of thread tid's register 0 under a layout (XOR of its bits' slots).
+__host__ __device__ __forceinline__ uint32_t slot_base(const LSlots& s, uint32_t tid) {
+  uint32_t b = 0;
 #pragma unroll
-    for (int j = 0; j < kRB; j++) sj[j] = uni32(lay[j] >= 0 ? swz<R>(1u << lay[j]) : 0u);
-  }
-  __host__ __device__ __forceinline__ uint32_t slot(int r) const {
-    uint32_t u = 0;
+  for (int i = 0; i < 8; i++)
+    if ((tid >> i) & 1) b ^= uni32(s.ts[i]);
+  return b;
+}
+
+// XOR of the register-position slots of register r (uniform across the workgroup)
+__host__ __device__ __forceinline__ uint32_t reg_slot(const LSlots& s, int r) {
+  uint32_t u = 0;
 #pragma unroll
-    for (int j = 0; j < kRB; j++)
-      if ((r >> j) & 1) u ^= sj[j];
-    return s0 ^ u;
-  }
-};
+  for (int j = 0; j < kRB; j++)
+    if ((r >> j) & 1) u ^= uni32(s.rs[j]);
+  return u;
+}
 
 template <typename R>
-__host__ __device__ __forceinline__ void regs_to_lds(const C<R>* v, C<R>* lds, const int8_t* lay, int T, uint32_t tid,
+__host__ __device__ __forceinline__ void regs_to_lds(const C<R>* v, C<R>* lds, const LSlots& S, uint32_t tid,
                                                      bool active, int nvalid_mask) {
   constexpr int NR = 1 << TileCfg<R>::RB;
-  const LdsSlots<R> S(lay, T, tid);
+  const uint32_t s0 = slot_base(S, tid);
   if (active && nvalid_mask == 0) {
 #pragma unroll
-    for (int r = 0; r < NR; r++) lds[S.slot(r)] = v[r];
+    for (int r = 0; r < NR; r++) lds[s0 ^ reg_slot(S, r)] = v[r];
   } else {
 #pragma unroll
     for (int r = 0; r < NR; r++)
-      if (active && !(r & nvalid_mask)) lds[S.slot(r)] = v[r];
+      if (active && !(r & nvalid_mask)) lds[s0 ^ reg_slot(S, r)] = v[r];
   }
 }
 
@@ -654,11 +681,11 @@ __host__ __device__ __forceinline__ void regs_to_lds(const C<R>* v, C<R>* lds, c
 // inactive threads pass a mirrored tid < 2^tb and phantom registers alias a
 // valid one; such values are never stored or reduced.
 template <typename R>
-__host__ __device__ __forceinline__ void lds_to_regs(C<R>* v, const C<R>* lds, const int8_t* lay, int T, uint32_t ltid) {
+__host__ __device__ __forceinline__ void lds_to_regs(C<R>* v, const C<R>* lds, const LSlots& S, uint32_t ltid) {
   constexpr int NR = 1 << TileCfg<R>::RB;
-  const LdsSlots<R> S(lay, T, ltid);
+  const uint32_t s0 = slot_base(S, ltid);
 #pragma unroll
-  for (int r = 0; r < NR; r++) v[r] = lds[S.slot(r)];
+  for (int r = 0; r < NR; r++) v[r] = lds[s0 ^ reg_slot(S, r)];
 }
 
 // Per-thread constants of a pass: offsets (in amplitudes, from the tile's
@@ -883,20 +910,23 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
     int8_t cur[kRB];
 #pragma unroll
     for (int j = 0; j < kRB; j++) cur[j] = g.io_in[j];
+    int ci = kLayIn;
     for (int p = 0; p <= g.nph; p++) {
       int8_t want[kRB];
+      const int wi = p < g.nph ? p : kLayOut;
 #pragma unroll
       for (int j = 0; j < kRB; j++) want[j] = p < g.nph ? g.ph[p].rb[j] : g.io_out[j];
       bool same = true;
 #pragma unroll
       for (int j = 0; j < kRB; j++) same = same && (cur[j] == want[j]);
       if (!same) {
-        regs_to_lds<R>(v, lds, cur, T, io.tid, io.active, io.nvalid_mask);
+        regs_to_lds<R>(v, lds, g.lay[ci], io.tid, io.active, io.nvalid_mask);
         __syncthreads();
-        lds_to_regs<R>(v, lds, want, T, io.ltid);
+        lds_to_regs<R>(v, lds, g.lay[wi], io.ltid);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kRB; j++) cur[j] = want[j];
+        ci = wi;
       }
       if (p == g.nph) break;
       const int op0 = g.ph[p].op0, op1 = op0 + g.ph[p].nops;
@@ -906,12 +936,12 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
         if (op.k <= 2) {
           apply_mask<R>(op.mask, v, mats + op.moff);
         } else {
-          regs_to_lds<R>(v, lds, cur, T, io.tid, io.active, io.nvalid_mask);
+          regs_to_lds<R>(v, lds, g.lay[ci], io.tid, io.active, io.nvalid_mask);
           __syncthreads();
           if (op.k == 3) apply_lds<R, 3>(lds, T, op, mats + op.moff, io.tid);
           else apply_lds<R, 4>(lds, T, op, mats + op.moff, io.tid);
           __syncthreads();
-          lds_to_regs<R>(v, lds, cur, T, io.ltid);
+          lds_to_regs<R>(v, lds, g.lay[ci], io.ltid);
           __syncthreads();
         }
       }

@@ -117,17 +117,20 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
     }
     int8_t cur[kRB];
     for (int j = 0; j < kRB; j++) cur[j] = g.io_in[j];
+    LSlots curS = g.lay[kLayIn];
     for (int p = 0; p <= g.nph; p++) {
       int8_t want[kRB];
       for (int j = 0; j < kRB; j++) want[j] = p < g.nph ? g.ph[p].rb[j] : g.io_out[j];
+      const LSlots wantS = g.lay[p < g.nph ? p : kLayOut];
       bool same = true;
       for (int j = 0; j < kRB; j++) same = same && (cur[j] == want[j]);
       if (!same) {
         if (t == 0) g_gstat[7]++;
         for (int tid = 0; tid < NT; tid++)
-          regs_to_lds<R>(v[tid].data(), lds.data(), cur, T, tid, io[tid].active, io[tid].nvalid_mask);
-        for (int tid = 0; tid < NT; tid++) lds_to_regs<R>(v[tid].data(), lds.data(), want, T, io[tid].ltid);
+          regs_to_lds<R>(v[tid].data(), lds.data(), curS, tid, io[tid].active, io[tid].nvalid_mask);
+        for (int tid = 0; tid < NT; tid++) lds_to_regs<R>(v[tid].data(), lds.data(), wantS, io[tid].ltid);
         for (int j = 0; j < kRB; j++) cur[j] = want[j];
+        curS = wantS;
       }
       if (p == g.nph) break;
       const int op0 = g.ph[p].op0, op1 = op0 + g.ph[p].nops;
@@ -139,12 +142,12 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
         } else {
           if (t == 0) g_gstat[8]++;
           for (int tid = 0; tid < NT; tid++)
-            regs_to_lds<R>(v[tid].data(), lds.data(), cur, T, tid, io[tid].active, io[tid].nvalid_mask);
+            regs_to_lds<R>(v[tid].data(), lds.data(), curS, tid, io[tid].active, io[tid].nvalid_mask);
           for (int tid = 0; tid < NT; tid++) {
             if (op.k == 3) apply_lds<R, 3>(lds.data(), T, op, mats.data() + op.moff, tid);
             else apply_lds<R, 4>(lds.data(), T, op, mats.data() + op.moff, tid);
           }
-          for (int tid = 0; tid < NT; tid++) lds_to_regs<R>(v[tid].data(), lds.data(), cur, T, io[tid].ltid);
+          for (int tid = 0; tid < NT; tid++) lds_to_regs<R>(v[tid].data(), lds.data(), curS, io[tid].ltid);
         }
       }
       if (p == g.nph - 1 && g.red == bti::RED_RHO) {
