@@ -256,7 +256,7 @@ def cpu_leg(args):
     L = int(max(1, min(args.layers, args.cpu_seconds // max(t1, 1e-3))))
     dt = run_layers(L)
     per_traj = dt * args.layers / L
-    return dict(value=round(1.0 / per_traj, 5), unit="trajectories/s", cores=O.threads(), kind="port",
+    return dict(value=round(1.0 / per_traj, 5), unit="trajectories/s", cores=int(O.num_threads()), kind="port",
                 sample="first %d of %d layers of one trajectory of the same N=%d circuit (%.1f s), scaled to a "
                        "whole trajectory; FP64 in-place strided kernels, OpenMP" % (L, args.layers, args.N, dt))
 

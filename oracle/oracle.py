@@ -496,8 +496,3 @@ def weighted_sample(probs, u):
 
 def num_threads():
     return lib().bto_num_threads()
-
-
-def threads():
-    """OpenMP threads the oracle's fast mode uses."""
-    return int(lib().bto_num_threads())
