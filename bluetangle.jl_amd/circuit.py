@@ -230,6 +230,11 @@ class Circuit:
         nn = min(cap, nb.value // norm_every) if norm_every > 0 else 0
         return d, norms[:nn].tolist()
 
+    def capture(self, state: StateVector, max_k=5):
+        """The fused form of this (noiseless, gate-only) circuit captured as a
+        HIP graph bound to `state`; call .launch() to run it again."""
+        return FusedGraph(self, state, max_k)
+
     def autotune(self, prec="c64", sample=64, seed=12345):
         """Pick the batched executor's tile bits, coalescing reserve and tiles
         per workgroup by timing `sample` trajectories of this circuit per
@@ -293,6 +298,34 @@ def run(ops_or_circuit, number_of_experiment: int = 1, N=None, noise=False, seed
         o, _ = c.run_on(st, seed, t)
         res.append(o)
     return res
+
+
+class FusedGraph:
+    """A captured fused circuit (bt_graph_*): launch() enqueues every pass at
+    once on the state's stream; state.sync() waits."""
+
+    def __init__(self, circuit: Circuit, state: StateVector, max_k=5):
+        h = ctypes.c_void_p()
+        L.check(L.lib().bt_graph_create_fused(state.handle, circuit.handle, max_k, ctypes.byref(h)))
+        self.handle = h
+        self.state = state  # the graph is bound to this state's memory
+        self.passes = int(L.lib().bt_graph_passes(h))
+
+    def launch(self):
+        L.check(L.lib().bt_graph_launch(self.handle))
+        return self.state
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h is not None and L._lib is not None:
+            L._lib.bt_graph_destroy(h)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

@@ -283,3 +283,94 @@ extern "C" int bt_run_fused(bt_state s, bt_circuit c, int max_k, int norm_every,
   if (h_al) (void)hipHostFree(h_al);
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// HIP graphs of fused circuits: the whole pass sequence captured once and
+// replayed with one launch (no per-pass host work).  The graph owns its
+// matrices; it is bound to the state it was captured on.
+// ---------------------------------------------------------------------------
+struct bt_graph_s {
+  bt_state_s* s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  float* d_al = nullptr;
+  std::vector<btf::Block> blocks;
+  int64_t nblocks = 0;
+};
+
+extern "C" int bt_graph_create_fused(bt_state s, bt_circuit c, int max_k, bt_graph* out) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (!s || !c || !out) return fail(BT_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (s->kind != BT_SV) return fail(BT_ERR_ARG, "bt_graph_create_fused: state vectors only");
+  if (s->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
+  const bool mfma = s->prec == BT_C64 && max_k == 5 && s->n >= kF5T;
+  if (!mfma && max_k > 3)
+    return fail(BT_ERR_UNSUPPORTED, "bt_graph_create_fused: complex64 with max_k = 5 (MFMA) or max_k <= 3");
+  auto* g = new bt_graph_s();
+  g->s = s;
+  std::string msg;
+  rc = btf::fuse_gates(c->h, max_k, mfma ? 5 : 0, &g->blocks, &msg);
+  if (rc) {
+    delete g;
+    return fail(rc, msg);
+  }
+  g->nblocks = (int64_t)g->blocks.size();
+  const size_t per = 64 * 64;
+  auto cleanup = [&](int r) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->d_al) (void)hipFree(g->d_al);
+    delete g;
+    return r;
+  };
+  if (mfma && g->nblocks) {
+    std::vector<float> al(per * g->blocks.size());
+    for (size_t b = 0; b < g->blocks.size(); b++)
+      fused5_lane_matrix((const double*)g->blocks[b].m.data(), al.data() + per * b);
+    if (hipMalloc((void**)&g->d_al, sizeof(float) * al.size()) != hipSuccess)
+      return cleanup(fail(BT_ERR_NOMEM, "bt_graph_create_fused: matrix buffer"));
+    if (hipMemcpy(g->d_al, al.data(), sizeof(float) * al.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(BT_ERR_HIP, "bt_graph_create_fused: matrix upload"));
+  }
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return cleanup(fail(BT_ERR_HIP, "sync"));
+  if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return cleanup(fail(BT_ERR_HIP, "bt_graph_create_fused: begin capture"));
+  int crc = BT_OK;
+  for (size_t b = 0; crc == BT_OK && b < g->blocks.size(); b++) {
+    const btf::Block& blk = g->blocks[b];
+    if (mfma) {
+      crc = launch_fused5(s, blk.bits, g->d_al + per * b);
+    } else {
+      Placement p;
+      crc = make_placement(s->nbits, blk.k, blk.bits, 0, &p);
+      if (crc == BT_OK) crc = launch_apply(s, s->nbits, blk.k, (const double*)blk.m.data(), p, 1.0, false);
+    }
+  }
+  const hipError_t ec = hipStreamEndCapture(s->stream, &g->graph);
+  if (crc) return cleanup(crc);
+  if (ec != hipSuccess || !g->graph) return cleanup(fail(BT_ERR_HIP, "bt_graph_create_fused: end capture"));
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    return cleanup(fail(BT_ERR_HIP, "bt_graph_create_fused: instantiate"));
+  *out = g;
+  return BT_OK;
+}
+
+extern "C" int bt_graph_launch(bt_graph g) {
+  if (!g || !g->exec) return fail(BT_ERR_ARG, "null graph");
+  BT_HIP(hipGraphLaunch(g->exec, g->s->stream));
+  return BT_OK;
+}
+
+extern "C" int64_t bt_graph_passes(bt_graph g) { return g ? g->nblocks : -1; }
+
+extern "C" int bt_graph_destroy(bt_graph g) {
+  if (!g) return BT_OK;
+  if (g->s && g->s->stream) (void)hipStreamSynchronize(g->s->stream);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->d_al) (void)hipFree(g->d_al);
+  delete g;
+  return BT_OK;
+}

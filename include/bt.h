@@ -54,6 +54,7 @@ extern "C" {
 typedef struct bt_state_s* bt_state;
 typedef struct bt_circuit_s* bt_circuit;
 typedef struct bt_server_s* bt_server;
+typedef struct bt_graph_s* bt_graph;
 
 /* ---- library ------------------------------------------------------------ */
 const char* bt_last_error(void);
@@ -192,6 +193,15 @@ int bt_apply_fused5(bt_state s, const double* mat, const int* bits);
  * *nblocks: blocks applied. */
 int bt_run_fused(bt_state s, bt_circuit c, int max_k, int norm_every, double* norms, int64_t nnorms, int64_t* nblocks,
                  bt_run_stats* stats);
+
+/* A fused noiseless circuit captured as a HIP graph on state s: every pass
+ * of bt_run_fused recorded once and replayed by bt_graph_launch (one launch,
+ * enqueued on s's stream; bt_state_sync waits).  The graph is bound to s
+ * and owns its matrices.  complex64 with max_k = 5, or max_k <= 3. */
+int bt_graph_create_fused(bt_state s, bt_circuit c, int max_k, bt_graph* out);
+int bt_graph_launch(bt_graph g);
+int64_t bt_graph_passes(bt_graph g);
+int bt_graph_destroy(bt_graph g);
 
 /* Tile width (bits held in LDS per pass) of the circuit executors for a
  * precision; 0 restores the default (14 for BT_C64, 13 for BT_C128).  Set by

@@ -99,3 +99,26 @@ def test_inner_across_precisions(bt):
     assert abs(a128.inner(b128) - np.vdot(v, w)) < 1e-9 * np.abs(v).sum()
     assert abs(a64.inner(a128) - np.vdot(v, v)) < 1e-5 * np.vdot(v, v).real
     assert abs(a128.inner(a128).real - a128.norm2()) < 1e-9
+
+
+@pytest.mark.parametrize("prec,max_k", [("c64", 5), ("c64", 3), ("c128", 3)])
+def test_captured_graph_replays(bt, prec, max_k):
+    """A captured fused circuit replayed several times, with eager work on the
+    same stream between replays, matches the eager runs each time."""
+    from bluetangle_jl_amd.circuit import Circuit
+    N = 14
+    spec = brickwork_spec(N, 6, 8, gate_noise=False)
+    c = Circuit(to_bt(spec, bt), N)
+    eager = bt.zero_state(N, prec)
+    g_state = bt.zero_state(N, prec)
+    g = c.capture(g_state, max_k=max_k)
+    assert g.passes > 0
+    tol = 1e-5 if prec == "c64" else 1e-12
+    for rep in range(3):
+        c.run_fused(eager, max_k=max_k)      # eager reference, applied once more each round
+        g.launch()
+        g_state.sync()
+        n = g_state.norm2()                  # eager work between replays
+        assert abs(n - eager.norm2()) < 10 * tol
+        assert np.abs(g_state.to_numpy() - eager.to_numpy()).max() < 10 * tol, rep
+    g.close()
