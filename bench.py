@@ -80,6 +80,20 @@ def c3_leg(bt, args):
     return c, cfg, stats, dt, tuning
 
 
+def precision_leg(bt, c, args):
+    """Trajectory 0 of the benchmark circuit in complex64 against the same
+    trajectory in complex128 (FP64 mode, validated against the oracle in the
+    tests): max amplitude error, fidelity, identical outcomes."""
+    import numpy as np
+    a, b = bt.zero_state(c.N, "c64"), bt.zero_state(c.N, "c128")
+    oa, _ = c.run_on(a, seed=1, traj=0)
+    ob, _ = c.run_on(b, seed=1, traj=0)
+    err = float(np.abs(a.to_numpy() - b.to_numpy()).max())
+    fid = abs(a.inner(b)) ** 2 / (a.norm2() * b.norm2())
+    return dict(max_amp_err_vs_fp64=err, fidelity_vs_fp64=fid, outcomes_identical=oa == ob,
+                sample="trajectory 0 (seed 1) of the benchmark circuit")
+
+
 def plain_loop_leg(bt, c, args):
     st = bt.zero_state(c.N, "c64")
     n = args.plain_traj
@@ -329,6 +343,8 @@ def main():
                      "launches": int(launches), "avg_launch_ms": round(kms / max(1, launches), 4),
                      "bytes_per_launch": round(nbytes / max(1, launches), 1)},
     }
+    out["precision"] = precision_leg(bt, c, args)
+    log("precision", out["precision"])
     if "plain" not in skip:
         pl = plain_loop_leg(bt, c, args)
         out["plain_loop"] = {"value": round(pl, 3), "unit": "trajectories/s", "sample": args.plain_traj,

@@ -317,7 +317,9 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
       break;
     }
     (void)hipEventRecord(ev[slot], s->stream);
-    const bool narrow = nbits <= 32;
+    // 32-bit in-tile offsets whenever every tile bit is below 32 (the tile
+    // base, which holds the outside bits, stays 64-bit)
+    const bool narrow = plan.ntile == 0 || plan.tile[plan.ntile - 1] < 32;
     if (s->prec == BT_C64) {
       if (narrow)
         hipLaunchKernelGGL((k_grid_pass<float, uint32_t>), dim3(blocks), dim3(TileCfg<float>::NT), lds, s->stream,

@@ -72,13 +72,16 @@ __global__ void __launch_bounds__(256, 2) k_fused5(F5Args a, const float* __rest
 #pragma unroll
   for (int b = 0; b < 4; b++) dhi[b] = 1ull << a.tile[8 + b];
   const bool hi_half = lane >= 32;
-  for (uint64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-    uint64_t base = 0, tt = t;
+  auto tile_base = [&](uint64_t t) {
+    uint64_t base = 0;
     for (int r = 0; r < a.nrun; r++) {
-      base |= (tt & ((1ull << a.run_len[r]) - 1ull)) << a.run_dst[r];
-      tt >>= a.run_len[r];
+      base |= (t & ((1ull << a.run_len[r]) - 1ull)) << a.run_dst[r];
+      t >>= a.run_len[r];
     }
-    float2 x[16];
+    return base;
+  };
+  float2 x[16];
+  auto fetch = [&](uint64_t base) {
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       uint64_t gi = base | dlo;
@@ -87,8 +90,14 @@ __global__ void __launch_bounds__(256, 2) k_fused5(F5Args a, const float* __rest
         if ((i >> b) & 1) gi |= dhi[b];
       x[i] = a.s[gi];
     }
+  };
+  if (blockIdx.x < a.ntiles) fetch(tile_base(blockIdx.x));
+  for (uint64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const uint64_t base = tile_base(t);
 #pragma unroll
     for (int i = 0; i < 16; i++) lds[f5_swz(tid + 256u * i)] = x[i];
+    // the next tile's loads fly during this tile's MFMA work (tiles are disjoint)
+    if (t + gridDim.x < a.ntiles) fetch(tile_base(t + gridDim.x));
     __syncthreads();
     float bv[32];
 #pragma unroll
