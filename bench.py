@@ -21,7 +21,8 @@ Beside it, on the same JSON line:
                 launch per pass, one circuit per launch);
   per_op_loop   the plain form: apply(state, op; noise) op by op;
   gate_layer    single-gate layers on an N=30 complex64 state (per-op
-                kernels), achieved GB/s against 8 TB/s;
+                kernels), achieved GB/s against 8 TB/s; then expect(state, "Z")
+                and expect(state, "X") over every qubit of that state;
   serving       Poisson arrivals of mixed circuits through the continuous-
                 batching server: p50/p99 latency and throughput;
   large_state   N=33 complex64 brickwork: 5-qubit blocks on FP32 MFMA vs the
@@ -134,11 +135,29 @@ def gate_layer_leg(bt, args):
     dt = time.perf_counter() - t0
     nlayers = reps * len(ops)
     gbs = nlayers * 2.0 * (1 << N) * 8 / dt / 1e9
+    obs = observables_on(bt, st, N)
     del st
     return dict(N=N, dtype="complex64", layers=nlayers, ms_per_layer=round(dt / nlayers * 1e3, 4),
                 achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
                 note="one Hadamard per layer on each qubit in turn; read+write of the state per layer; wall time "
-                     "of back-to-back launches")
+                     "of back-to-back launches", observables=obs)
+
+
+def observables_on(bt, st, N):
+    """expect(state, "Z") and expect(state, "X") over every qubit of the
+    resident state: one FP64 reduction (one read of the state) per qubit."""
+    out = {}
+    for name in ("Z", "X"):
+        bt.expect(st, name)
+        t0 = time.perf_counter()
+        vals = bt.expect(st, name)
+        dt = time.perf_counter() - t0
+        gbs = N * (1 << N) * 8 / dt / 1e9
+        out[name] = dict(ms_per_qubit=round(dt / N * 1e3, 4), achieved_gbs=round(gbs, 1),
+                         frac=round(gbs / HBM_PEAK_GBS, 4), mean=round(float(sum(vals)) / N, 6))
+    out["note"] = ("one read of the 2^N complex64 state per qubit (bt_expect), wall time including the host "
+                   "round trip of each result")
+    return out
 
 
 def serving_leg(bt, args):

@@ -1,7 +1,7 @@
 """MI355X-native engine for BlueTangle.jl's noisy-circuit apply path.
 
 Mirrors the reference's Julia API (Op, OpQC, ifOp, OpF, QuantumChannel,
-NoiseModel, apply, to_state, to_rho, run, sample_exact) over the C ABI of
+NoiseModel, apply, to_state, to_rho, run, sample_exact, expect, correlation) over the C ABI of
 libbt.so (include/bt.h).  There is no CPU fallback.
 """
 from ._lib import BTError
@@ -12,11 +12,12 @@ from .ops import Op, OpF, OpQC, QuantumOps, ifOp
 from .core import (DensityMatrix, DeviceState, StateVector, UniformStream, apply, apply_noise, current_stream,
                    from_numpy, get_N, init, sample_exact, seed, zero_rho, zero_state)
 from .circuit import Circuit, Server, compile_ir, run, to_rho, to_state
+from .observables import correlation, expand_multi_op, expect
 
 __all__ = [
     "BTError", "gate", "gates", "Noise1", "Noise2", "NoiseModel", "QuantumChannel", "custom_noise",
     "is_valid_quantum_channel", "noise_model", "noise_model1", "noise_model2", "Op", "OpF", "OpQC", "QuantumOps",
     "ifOp", "DensityMatrix", "DeviceState", "StateVector", "UniformStream", "apply", "apply_noise",
-    "current_stream", "from_numpy", "get_N", "init", "sample_exact", "seed", "zero_rho", "zero_state",
+    "current_stream", "correlation", "expand_multi_op", "expect", "from_numpy", "get_N", "init", "sample_exact", "seed", "zero_rho", "zero_state",
     "Circuit", "Server", "compile_ir", "run", "to_rho", "to_state",
 ]

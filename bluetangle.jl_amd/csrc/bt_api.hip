@@ -581,6 +581,15 @@ int bt_inner(bt_state a, bt_state b, double* re, double* im) {
   return launch_inner(a, b, re, im);
 }
 
+int bt_expect(bt_state s, int k, const double* mat, const int* bits, uint64_t xmask, uint64_t zmask, double* re,
+              double* im) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!re || !im) return fail(BT_ERR_ARG, "null output");
+  if (k > 0 && (!mat || !bits)) return fail(BT_ERR_ARG, "bt_expect: null block");
+  return launch_expect(s, k, mat, bits, xmask, zmask, re, im);
+}
+
 int bt_trace_purity(bt_state s, double* trace, double* purity) {
   int rc = check_state(s);
   if (rc) return rc;

@@ -139,6 +139,19 @@ int launch_reduce(bt_state_s* s, int k, const int* bits);
 void unpack_reduced(int k, const double* packed, double* rho_colmajor);
 int launch_norm2(bt_state_s* s, double* out);
 int launch_inner(bt_state_s* a, bt_state_s* b, double* re, double* im);  // <a|b>, FP64
+// Expectation of a Pauli string (x, z masks) times a dense k-bit block (k <= 3)
+// on a state vector (<psi|O|psi>) or density matrix (tr(rho O)), FP64.
+struct ExpectArgs {
+  uint64_t ngroups;  // 2^(n - nfixed)
+  uint64_t xmask, zmask;
+  uint64_t off[8];   // offset of block index l (bits[0] = MSB)
+  int32_t n;         // qubits
+  int32_t nfixed;    // inserted bits: the block's, plus the top bit of x on a state vector
+  int32_t fixed[4];  // ascending
+  double g[2 * 64];  // block, complex128 column-major
+};
+int launch_expect(bt_state_s* s, int k, const double* mat, const int* bits, uint64_t xmask, uint64_t zmask,
+                  double* re, double* im);
 int launch_trace(bt_state_s* s, double* out);
 int launch_probabilities(bt_state_s* s, double* d_out);
 int launch_scale(bt_state_s* s, double f);

@@ -92,34 +92,53 @@ struct ReduceArgs {
 
 // Packed layout of a reduced rho (D*D reals): [0, D) diagonal, then for
 // each a < b (row-major over a): Re, Im of rho[a, b] = sum v_a conj(v_b).
+// Items per loop trip of the reductions: their loads are issued together
+// (more bytes in flight per wave); an item past the end re-reads the trip's
+// first item and contributes nothing.
+template <int K>
+constexpr int red_unroll() { return K == 0 ? 4 : K == 1 ? 2 : 1; }
+
 template <typename R, int K>
 __global__ void __launch_bounds__(256) k_reduce(ReduceArgs<R, K> a) {
   constexpr int D = 1 << K;
   constexpr int M = D * D;
+  constexpr int U = red_unroll<K>();
   double acc[M];
 #pragma unroll
   for (int j = 0; j < M; j++) acc[j] = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.ngroups; g += stride) {
-    uint64_t base = g;
-    for (int i = 0; i < a.nfixed; i++) base = insert0(base, a.fixed[i]);
-    double vr[D], vi[D];
+  for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g0 < a.ngroups; g0 += U * stride) {
+    C<R> v[U][D];
+    bool live[U];
 #pragma unroll
-    for (int l = 0; l < D; l++) {
-      C<R> v = a.s[base + a.off[l]];
-      vr[l] = (double)v.x;
-      vi[l] = (double)v.y;
+    for (int u = 0; u < U; u++) {
+      const uint64_t g = g0 + u * stride;
+      live[u] = g < a.ngroups;
+      uint64_t base = live[u] ? g : g0;
+      for (int i = 0; i < a.nfixed; i++) base = insert0(base, a.fixed[i]);
+#pragma unroll
+      for (int l = 0; l < D; l++) v[u][l] = a.s[base + a.off[l]];
     }
-    int p = D;
 #pragma unroll
-    for (int x = 0; x < D; x++) {
-      acc[x] = fma(vr[x], vr[x], fma(vi[x], vi[x], acc[x]));
+    for (int u = 0; u < U; u++) {
+      if (!live[u]) continue;
+      double vr[D], vi[D];
 #pragma unroll
-      for (int y = x + 1; y < D; y++) {
-        // v_x conj(v_y)
-        acc[p] += vr[x] * vr[y] + vi[x] * vi[y];
-        acc[p + 1] += vi[x] * vr[y] - vr[x] * vi[y];
-        p += 2;
+      for (int l = 0; l < D; l++) {
+        vr[l] = (double)v[u][l].x;
+        vi[l] = (double)v[u][l].y;
+      }
+      int p = D;
+#pragma unroll
+      for (int x = 0; x < D; x++) {
+        acc[x] = fma(vr[x], vr[x], fma(vi[x], vi[x], acc[x]));
+#pragma unroll
+        for (int y = x + 1; y < D; y++) {
+          // v_x conj(v_y)
+          acc[p] += vr[x] * vr[y] + vi[x] * vi[y];
+          acc[p + 1] += vi[x] * vr[y] - vr[x] * vi[y];
+          p += 2;
+        }
       }
     }
   }
@@ -146,13 +165,116 @@ __global__ void __launch_bounds__(256) k_finalize(const double* part, int nblock
 // per-workgroup partials (re, im)
 template <typename RA, typename RB>
 __global__ void __launch_bounds__(256) k_inner(const C<RA>* a, const C<RB>* b, uint64_t n, double* part) {
+  constexpr int U = 4;
   double acc[2] = {0.0, 0.0};
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const C<RA> x = a[i];
-    const C<RB> y = b[i];
-    acc[0] = fma((double)x.x, (double)y.x, fma((double)x.y, (double)y.y, acc[0]));
-    acc[1] = fma((double)x.x, (double)y.y, fma(-(double)x.y, (double)y.x, acc[1]));
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += U * stride) {
+    C<RA> x[U];
+    C<RB> y[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t i = i0 + u * stride < n ? i0 + u * stride : i0;
+      x[u] = a[i];
+      y[u] = b[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (i0 + u * stride >= n) continue;
+      acc[0] = fma((double)x[u].x, (double)y[u].x, fma((double)x[u].y, (double)y[u].y, acc[0]));
+      acc[1] = fma((double)x[u].x, (double)y[u].y, fma(-(double)x[u].y, (double)y[u].x, acc[1]));
+    }
+  }
+  block_reduce_store<2>(acc, part + (size_t)blockIdx.x * 2);
+}
+
+// Expectation of O = P(x, z) G(B): P(x, z)[a, b] = (-1)^popcount(a & z) when
+// b = a ^ x, G a dense 2^K x 2^K block on the bits B (disjoint from x and z),
+// identity elsewhere.  Work item g enumerates the indices a0 with the B bits
+// clear and sums over a = a0 | off(i), b = (a0 ^ x) | off(j):
+//   SV: conj(psi_a) O[a, b] psi_b       DM: O[a, b] rho[b + a 2^n]
+// On a state vector with x != 0, a0 also has the top bit of x clear and the
+// item takes both a0 and a0 ^ x, so each amplitude is read once.
+// Per-workgroup FP64 partials (re, im).
+template <typename R, int D>
+__device__ __forceinline__ void expect_quad(const C<R> (&u)[D], const C<R> (&v)[D], const double* g, double& sr,
+                                            double& si) {
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    double tr = 0.0, ti = 0.0;  // (G v)_i
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      const double gr = g[2 * (i + j * D)], gi = g[2 * (i + j * D) + 1];
+      tr = fma(gr, (double)v[j].x, fma(-gi, (double)v[j].y, tr));
+      ti = fma(gr, (double)v[j].y, fma(gi, (double)v[j].x, ti));
+    }
+    sr = fma((double)u[i].x, tr, fma((double)u[i].y, ti, sr));
+    si = fma((double)u[i].x, ti, fma(-(double)u[i].y, tr, si));
+  }
+}
+
+template <typename R, int K, bool DM>
+__global__ void __launch_bounds__(256) k_expect(const C<R>* s, ExpectArgs a, double* part) {
+  constexpr int D = 1 << K;
+  constexpr int U = DM ? 1 : red_unroll<K>();
+  double acc[2] = {0.0, 0.0};
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g0 < a.ngroups; g0 += U * stride) {
+    if constexpr (!DM) {
+      C<R> u[U][D], v[U][D];
+      uint64_t a0[U];
+#pragma unroll
+      for (int t = 0; t < U; t++) {
+        const uint64_t g = g0 + t * stride < a.ngroups ? g0 + t * stride : g0;
+        a0[t] = g;
+        for (int j = 0; j < a.nfixed; j++) a0[t] = insert0(a0[t], a.fixed[j]);
+        const uint64_t b0 = a0[t] ^ a.xmask;
+#pragma unroll
+        for (int j = 0; j < D; j++) v[t][j] = s[b0 | a.off[j]];
+        if (a.xmask) {
+#pragma unroll
+          for (int j = 0; j < D; j++) u[t][j] = s[a0[t] | a.off[j]];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < U; t++) {
+        if (g0 + t * stride >= a.ngroups) continue;
+        if (!a.xmask) {
+#pragma unroll
+          for (int j = 0; j < D; j++) u[t][j] = v[t][j];
+        }
+        double sr = 0.0, si = 0.0;
+        expect_quad<R, D>(u[t], v[t], a.g, sr, si);
+        const bool neg = __popcll(a0[t] & a.zmask) & 1;
+        acc[0] += neg ? -sr : sr;
+        acc[1] += neg ? -si : si;
+        if (a.xmask) {  // the partner item a0 ^ x
+          double tr = 0.0, ti = 0.0;
+          expect_quad<R, D>(v[t], u[t], a.g, tr, ti);
+          const bool negb = __popcll((a0[t] ^ a.xmask) & a.zmask) & 1;
+          acc[0] += negb ? -tr : tr;
+          acc[1] += negb ? -ti : ti;
+        }
+      }
+    } else {
+      uint64_t a0 = g0;
+      for (int j = 0; j < a.nfixed; j++) a0 = insert0(a0, a.fixed[j]);
+      const uint64_t b0 = a0 ^ a.xmask;
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int i = 0; i < D; i++) {
+        const uint64_t col = (a0 | a.off[i]) << a.n;
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+          const C<R> r = s[(b0 | a.off[j]) + col];
+          const double gr = a.g[2 * (i + j * D)], gi = a.g[2 * (i + j * D) + 1];
+          sr = fma(gr, (double)r.x, fma(-gi, (double)r.y, sr));
+          si = fma(gr, (double)r.y, fma(gi, (double)r.x, si));
+        }
+      }
+      const bool neg = __popcll(a0 & a.zmask) & 1;
+      acc[0] += neg ? -sr : sr;
+      acc[1] += neg ? -si : si;
+    }
   }
   block_reduce_store<2>(acc, part + (size_t)blockIdx.x * 2);
 }
@@ -208,6 +330,13 @@ __global__ void k_gather(const C<R>* s, uint64_t len, const uint64_t* idx, uint6
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// Workgroups of a reduction with M partial values each: up to 4096 (eight
+// resident per CU), within the partial buffer of kRedBlocks * kRedMax values.
+static uint64_t red_cap(int M) {
+  const uint64_t c = (uint64_t)kRedBlocks * kRedMax / (uint64_t)M;
+  return c < 4096 ? c : 4096;
+}
+
 static unsigned grid_for(uint64_t work, unsigned block = 256, uint64_t cap = (1ull << 20)) {
   uint64_t g = (work + block - 1) / block;
   if (g < 1) g = 1;
@@ -298,7 +427,7 @@ static int reduce_t(bt_state_s* s, const Placement& p) {
   a.nfixed = p.nfixed;
   for (int i = 0; i < p.nfixed; i++) a.fixed[i] = p.fixed[i];
   a.part = s->d_part;
-  const unsigned blocks = grid_for(a.ngroups, 256, kRedBlocks);
+  const unsigned blocks = grid_for(a.ngroups, 256, red_cap(M));
   hipLaunchKernelGGL((k_reduce<R, K>), dim3(blocks), dim3(256), 0, s->stream, a);
   BT_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finalize, dim3(M), dim3(256), 0, s->stream, (const double*)s->d_part, (int)blocks, M, s->d_out);
@@ -310,7 +439,7 @@ static int reduce_t(bt_state_s* s, const Placement& p) {
 
 int launch_inner(bt_state_s* a, bt_state_s* b, double* re, double* im) {
   const uint64_t n = a->len;
-  const unsigned blocks = grid_for(n, 256, kRedBlocks);
+  const unsigned blocks = grid_for(n, 256, red_cap(2));
   const bool fa = a->prec == BT_C64, fb = b->prec == BT_C64;
   if (fa && fb)
     hipLaunchKernelGGL((k_inner<float, float>), dim3(blocks), dim3(256), 0, a->stream, (const C<float>*)a->d,
@@ -431,6 +560,71 @@ int launch_gather(bt_state_s* s, const uint64_t* d_idx, uint64_t n, double* d_ou
     hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, s->stream, (const C<double>*)s->d, s->len,
                        d_idx, n, d_out);
   BT_HIP(hipGetLastError());
+  return BT_OK;
+}
+
+template <typename R, int K>
+static void expect_t(bt_state_s* s, const ExpectArgs& a, unsigned blocks) {
+  if (s->kind == BT_DM)
+    hipLaunchKernelGGL((k_expect<R, K, true>), dim3(blocks), dim3(256), 0, s->stream, (const C<R>*)s->d, a, s->d_part);
+  else
+    hipLaunchKernelGGL((k_expect<R, K, false>), dim3(blocks), dim3(256), 0, s->stream, (const C<R>*)s->d, a,
+                       s->d_part);
+}
+
+int launch_expect(bt_state_s* s, int k, const double* mat, const int* bits, uint64_t xmask, uint64_t zmask,
+                  double* re, double* im) {
+  if (k < 0 || k > 3) return fail(BT_ERR_ARG, "expectation: dense block width k must be 0..3");
+  const int n = s->n;
+  const uint64_t all = n == 64 ? ~0ull : ((1ull << n) - 1ull);
+  if ((xmask | zmask) & ~all) return fail(BT_ERR_ARG, "expectation: Pauli mask outside the register");
+  ExpectArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n;
+  a.xmask = xmask;
+  a.zmask = zmask;
+  uint64_t used = 0;
+  for (int i = 0; i < k; i++) {
+    if (bits[i] < 0 || bits[i] >= n) return fail(BT_ERR_ARG, "expectation: bit out of range");
+    if ((used >> bits[i]) & 1) return fail(BT_ERR_ARG, "expectation: repeated bit");
+    used |= 1ull << bits[i];
+  }
+  if (used & (xmask | zmask)) return fail(BT_ERR_ARG, "expectation: dense bits overlap the Pauli masks");
+  // inserted bits, ascending: the block bits, and on a state vector the top bit of x
+  uint64_t ins = used;
+  if (s->kind == BT_SV && xmask) ins |= 1ull << (63 - __builtin_clzll(xmask));
+  int j = 0;
+  for (int b = 0; b < 64; b++)
+    if ((ins >> b) & 1) a.fixed[j++] = b;
+  a.nfixed = j;
+  const int D = 1 << k;
+  for (int l = 0; l < D; l++) {
+    uint64_t o = 0;
+    for (int i = 0; i < k; i++)
+      if ((l >> (k - 1 - i)) & 1) o |= 1ull << bits[i];
+    a.off[l] = o;
+  }
+  if (k == 0) {
+    a.g[0] = 1.0;
+  } else {
+    for (int i = 0; i < 2 * D * D; i++) a.g[i] = mat[i];
+  }
+  a.ngroups = 1ull << (n - a.nfixed);
+  const unsigned blocks = grid_for(a.ngroups, 256, red_cap(2));
+  const bool f = s->prec == BT_C64;
+  switch (k) {
+    case 0: f ? expect_t<float, 0>(s, a, blocks) : expect_t<double, 0>(s, a, blocks); break;
+    case 1: f ? expect_t<float, 1>(s, a, blocks) : expect_t<double, 1>(s, a, blocks); break;
+    case 2: f ? expect_t<float, 2>(s, a, blocks) : expect_t<double, 2>(s, a, blocks); break;
+    default: f ? expect_t<float, 3>(s, a, blocks) : expect_t<double, 3>(s, a, blocks); break;
+  }
+  BT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(2), dim3(256), 0, s->stream, (const double*)s->d_part, (int)blocks, 2, s->d_out);
+  BT_HIP(hipGetLastError());
+  BT_HIP(hipMemcpyAsync(s->h_out, s->d_out, sizeof(double) * 2, hipMemcpyDeviceToHost, s->stream));
+  BT_HIP(hipStreamSynchronize(s->stream));
+  *re = s->h_out[0];
+  *im = s->h_out[1];
   return BT_OK;
 }
 

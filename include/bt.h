@@ -117,6 +117,18 @@ int bt_norm2(bt_state s, double* out);
  * shape): fidelity |<a|b>|^2 / (<a|a> <b|b>) of a complex64 result against
  * its complex128 counterpart. */
 int bt_inner(bt_state a, bt_state b, double* re, double* im);
+/* Expectation value of O = P(xmask, zmask) * G, identity elsewhere, in FP64:
+ * <psi|O|psi> on a state vector, tr(rho O) on a density matrix (complex; the
+ * reference reports the real part).  P(x, z)[a, b] = (-1)^popcount(a & z) when
+ * b = a ^ x: bit b of x flips (X), of z signs (Z); Y on bit b is x and z with a
+ * factor -i for the caller.  G is a dense 2^k x 2^k complex128 column-major
+ * block on bits[0..k) (bits[0] = MSB, k <= 3, disjoint from x | z; mat may be
+ * null when k = 0).  Replaces expect(state|rho, op | op_str | matrix) and
+ * correlation(state|rho, op_str, qubits) (func.jl:91-100,142-151), whose
+ * operators expand_multi_op / op.expand build (ops.jl:928-944, struct.jl:476-548),
+ * and correlation(state, qubits) (func.jl:17-23: a Z string). */
+int bt_expect(bt_state s, int k, const double* mat, const int* bits, uint64_t xmask, uint64_t zmask, double* re,
+              double* im);
 /* DM: FP64 trace and purity sum |rho_ij|^2 */
 int bt_trace_purity(bt_state s, double* trace, double* purity);
 /* Amplitudes (or rho entries) at n indices, complex128 interleaved out. */
