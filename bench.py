@@ -157,6 +157,15 @@ def observables_on(bt, st, N):
                          frac=round(gbs / HBM_PEAK_GBS, 4), mean=round(float(sum(vals)) / N, 6))
     out["note"] = ("one read of the 2^N complex64 state per qubit (bt_expect), wall time including the host "
                    "round trip of each result")
+    import numpy as np
+    u = np.random.default_rng(1).random(1 << 20)
+    bt.measurement.sample_with(st, u[:1024])
+    t0 = time.perf_counter()
+    idx, nb = bt.measurement.sample_with(st, u)
+    dt = time.perf_counter() - t0
+    out["sample"] = dict(shots=int(u.size), ms=round(dt * 1e3, 3), boundary=int(nb),
+                         note="bt_sample: chunk sums (one read of the state), prefix, one wavefront per shot; "
+                              "wall time including the upload of the uniforms and the download of the indices")
     return out
 
 

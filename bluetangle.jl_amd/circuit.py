@@ -266,6 +266,26 @@ class Circuit:
                                      ctypes.byref(st)))
         return [outs[t, : nouts[t]].tolist() for t in range(ntraj)], states, st.as_dict()
 
+    def run_batch_sample(self, ntraj: int, seed=0, traj0=0, prec="c64", outcap=None, with_outcomes=False):
+        """run_batch, then one shot per trajectory from its final state (the
+        bt_sample rule, uniform u(seed, traj, draws used)).  Returns (indices,
+        boundary count[, outcome lists, stats])."""
+        from .core import PRECISIONS
+        outcap = outcap or max(16, 2 * self.nops)
+        outs = np.zeros((ntraj, outcap), dtype=np.int32)
+        nouts = np.zeros(ntraj, dtype=np.int64)
+        samples = np.zeros(ntraj, dtype=np.uint64)
+        nb = ctypes.c_int64()
+        st = L.RunStats()
+        L.check(L.lib().bt_run_batch_sample(self.handle, PRECISIONS[prec], seed, traj0, ntraj, L.iptr(outs), outcap,
+                                            nouts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                            samples.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                            ctypes.byref(nb), ctypes.byref(st)))
+        idx = samples.astype(np.int64)
+        if with_outcomes:
+            return idx, nb.value, [outs[t, : nouts[t]].tolist() for t in range(ntraj)], st.as_dict()
+        return idx, nb.value
+
 
 def to_state(ops_or_circuit, N=None, noise=False, seed=0, traj=0, prec="c64"):
     """to_state (ops.jl:790-795): one trajectory from |0..0>."""

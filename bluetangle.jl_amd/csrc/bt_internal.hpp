@@ -152,6 +152,13 @@ struct ExpectArgs {
 };
 int launch_expect(bt_state_s* s, int k, const double* mat, const int* bits, uint64_t xmask, uint64_t zmask,
                   double* re, double* im);
+// Shot sampling of a state vector (bt_sample.hip): shot s takes the smallest i
+// with u_s T < C_i (FP64 cumulative sums in index order, T the total);
+// d_u null -> every shot takes u0.  d_flag[s] = 1 when u_s T lies within 1e-12 T
+// of a boundary.  scratch: sample_scratch_bytes(nbits) device bytes.
+size_t sample_scratch_bytes(int nbits);
+int launch_sample(const void* state, int prec, int nbits, hipStream_t st, void* scratch, int64_t shots,
+                  const double* d_u, double u0, uint64_t* d_out, uint8_t* d_flag);
 int launch_trace(bt_state_s* s, double* out);
 int launch_probabilities(bt_state_s* s, double* d_out);
 int launch_scale(bt_state_s* s, double f);

@@ -513,8 +513,11 @@ static int export_state(Job* j, hipStream_t st, double* host, double* d_tmp) {
 
 using namespace bt;
 
-extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
-                            int64_t outcap, int64_t* nouts, double* final_states, bt_run_stats* stats) {
+// The lockstep-rounds batch; samples (nullable): one basis index per
+// trajectory drawn from its final state with u(seed, traj, draws used).
+static int run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
+                     int64_t outcap, int64_t* nouts, double* final_states, uint64_t* samples,
+                     int64_t* sample_boundary, bt_run_stats* stats) {
   if (!c) return fail(BT_ERR_ARG, "null circuit");
   if (ntraj <= 0 || !outcomes || !nouts || outcap < 0) return fail(BT_ERR_ARG, "bad batch arguments");
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
@@ -555,6 +558,21 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
     cleanup();
     return fail(BT_ERR_NOMEM, "bt_run_batch: export buffer");
   }
+  // final sampling: scratch for one state at a time (the stream orders them),
+  // one index and one boundary flag per trajectory
+  char* d_samp = nullptr;
+  const size_t samp_scratch = samples ? sample_scratch_bytes(nbits) : 0;
+  std::vector<uint8_t> fresh_end(samples ? ntraj : 0, 0);
+  if (samples) {
+    if (hipMalloc((void**)&d_samp, samp_scratch + 9 * (size_t)ntraj + 64) != hipSuccess) {
+      (void)hipGetLastError();
+      cleanup();
+      return fail(BT_ERR_NOMEM, "bt_run_batch: sampling buffers");
+    }
+    slot_mem.push_back(d_samp);
+  }
+  uint64_t* d_sidx = samples ? (uint64_t*)(d_samp + samp_scratch) : nullptr;
+  uint8_t* d_sflag = samples ? (uint8_t*)(d_samp + samp_scratch + 8 * (size_t)ntraj) : nullptr;
   std::vector<std::unique_ptr<Job>> jobs(ntraj);
   std::vector<int64_t> free_slots;
   for (int64_t s = nslots - 1; s >= 0; s--) free_slots.push_back(s);
@@ -607,6 +625,15 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
         const int erc = export_state(j, stream, final_states + (uint64_t)t * 2 * (1ull << nbits), d_tmp);
         if (erc) return erc;
       }
+      if (samples) {
+        if (j->fresh) {  // never written: |0..0>, index 0 with certainty
+          fresh_end[t] = 1;
+        } else {
+          const int src = launch_sample(j->state, prec, nbits, stream, d_samp, 1, nullptr,
+                                        uniform(seed, j->traj, (uint64_t)j->draws), d_sidx + t, d_sflag + t);
+          if (src) return src;
+        }
+      }
       free_slots.push_back(slot_of[t]);
       jobs[t].reset();
       done++;
@@ -630,6 +657,23 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
   }
   for (int ch = 0; ch < 2; ch++) (void)E[ch]->finish_round();
   (void)hipStreamSynchronize(stream);
+  if (rc == BT_OK && samples) {
+    std::vector<uint8_t> flags(ntraj);
+    if (hipMemcpy(samples, d_sidx, 8 * (size_t)ntraj, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(flags.data(), d_sflag, (size_t)ntraj, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(BT_ERR_HIP, "bt_run_batch: sample download");
+    } else {
+      int64_t nb = 0;
+      for (int64_t t = 0; t < ntraj; t++) {
+        if (fresh_end[t]) {
+          samples[t] = 0;
+          flags[t] = 0;
+        }
+        nb += flags[t];
+      }
+      if (sample_boundary) *sample_boundary = nb;
+    }
+  }
   // device time of the pass kernels (HIP events around each launch)
   acc.kernel_ms = eng0.pass_ms() + eng1.pass_ms();
   acc.launches = eng0.rounds() + eng1.rounds();
@@ -639,6 +683,18 @@ extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0
   if (first_err) return fail(first_err, "bt_run_batch: a trajectory failed (interpreter error " +
                                             std::to_string(first_err) + ")");
   return BT_OK;
+}
+
+extern "C" int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
+                            int64_t outcap, int64_t* nouts, double* final_states, bt_run_stats* stats) {
+  return run_batch(c, prec, seed, traj0, ntraj, outcomes, outcap, nouts, final_states, nullptr, nullptr, stats);
+}
+
+extern "C" int bt_run_batch_sample(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj,
+                                   int* outcomes, int64_t outcap, int64_t* nouts, uint64_t* samples,
+                                   int64_t* sample_boundary, bt_run_stats* stats) {
+  if (!samples) return fail(BT_ERR_ARG, "bt_run_batch_sample: null samples");
+  return run_batch(c, prec, seed, traj0, ntraj, outcomes, outcap, nouts, nullptr, samples, sample_boundary, stats);
 }
 
 // ---------------------------------------------------------------------------

@@ -180,9 +180,13 @@ def _expect_full(state: DeviceState, M) -> complex:
     return state.inner(tmp)
 
 
-def expect(state: DeviceState, what, qubit: Union[int, None] = None, pars=None):
+def expect(state: DeviceState, what=None, qubit: Union[int, None] = None, pars=None):
     """expect(state|rho, op) / expect(state|rho, op_str) -> vector over qubits /
-    expect(state|rho, matrix) (func.jl:91-100): the real part, as the reference."""
+    expect(state|rho, matrix) (func.jl:91-100): the real part, as the reference.
+    On a Measurement: <Z_q> from its sample (func.jl:39)."""
+    from .measurement import Measurement, expect as _m_expect
+    if isinstance(state, Measurement):
+        return _m_expect(state, what)
     N = state.N
     if isinstance(what, str):
         m = gates(what)
@@ -195,7 +199,10 @@ def expect(state: DeviceState, what, qubit: Union[int, None] = None, pars=None):
 def correlation(state: DeviceState, list_of_operators, qubits_applied=None) -> float:
     """correlation(state|rho, "Z,Z", [2, 4]) (func.jl:142-151), or
     correlation(state, qubits) = <prod Z_q> from the exact probabilities
-    (func.jl:17-23,166-182)."""
+    (func.jl:17-23,166-182); correlation(m::Measurement, qubits) (func.jl:14)."""
+    from .measurement import Measurement, correlation as _m_corr
+    if isinstance(state, Measurement):
+        return _m_corr(state, list_of_operators)
     N = state.N
     if qubits_applied is None:
         qubits = list(list_of_operators)

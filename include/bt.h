@@ -129,6 +129,14 @@ int bt_inner(bt_state a, bt_state b, double* re, double* im);
  * and correlation(state, qubits) (func.jl:17-23: a Z string). */
 int bt_expect(bt_state s, int k, const double* mat, const int* bits, uint64_t xmask, uint64_t zmask, double* re,
               double* im);
+/* Shot sampling of a state vector: shot s returns the smallest basis index i
+ * with u[s] * T < C_i, C_i the FP64 cumulative sum of |psi_j|^2 over j <= i in
+ * index order and T the total (the state need not be normalised).  u: shots
+ * uniforms in [0, 1); out: shots indices.  nboundary (nullable): shots whose
+ * u T lies within 1e-12 T of a cumulative boundary, where summation order may
+ * decide.  Replaces sample(state, shots) (ops.jl:46-60; StatsBase.sample
+ * with Weights, whose RNG the explicit uniforms stand in for). */
+int bt_sample(bt_state s, int64_t shots, const double* u, uint64_t* out, int64_t* nboundary);
 /* DM: FP64 trace and purity sum |rho_ij|^2 */
 int bt_trace_purity(bt_state s, double* trace, double* purity);
 /* Amplitudes (or rho entries) at n indices, complex128 interleaved out. */
@@ -187,6 +195,15 @@ int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats);
  * trajectories in one launch; their interpreters run on host threads. */
 int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes, int64_t outcap,
                  int64_t* nouts, double* final_states, bt_run_stats* stats);
+/* bt_run_batch, then one shot per trajectory: the basis index drawn from its
+ * final state by the bt_sample rule with the trajectory's next uniform
+ * u(seed, traj0 + t, draws used) -- the noisy branch of measure(circuit, shots)
+ * (ops.jl:609-621: to_state, final basis, sample(state, 1) per experiment;
+ * the final basis rotation is part of the circuit's ops).  samples: ntraj;
+ * sample_boundary (nullable): shots within 1e-12 of a cumulative boundary. */
+int bt_run_batch_sample(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
+                        int64_t outcap, int64_t* nouts, uint64_t* samples, int64_t* sample_boundary,
+                        bt_run_stats* stats);
 /* The same contract in ONE persistent launch: each workgroup owns a
  * trajectory, runs its interpreter on the device (lane 0) and takes the next
  * trajectory from an atomic counter. */
