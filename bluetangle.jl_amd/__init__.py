@@ -14,6 +14,8 @@ from .core import (DensityMatrix, DeviceState, StateVector, UniformStream, apply
 from .circuit import Circuit, Server, compile_ir, run, to_rho, to_state
 from .observables import correlation, expand_multi_op, expect
 from . import measurement
+from .compile import (CompiledCircuit, Layout, Options, apply_twirl, compile, get_layers, get_optimized_layers,
+                      get_stats, optimize_simple)
 from .measurement import (Measurement, get_probs_from_sample, measure, measure_circuit, sample, sample_state)
 
 __all__ = [
@@ -21,6 +23,7 @@ __all__ = [
     "is_valid_quantum_channel", "noise_model", "noise_model1", "noise_model2", "Op", "OpF", "OpQC", "QuantumOps",
     "ifOp", "DensityMatrix", "DeviceState", "StateVector", "UniformStream", "apply", "apply_noise",
     "current_stream", "correlation", "Measurement", "get_probs_from_sample", "measure", "measure_circuit",
-    "measurement", "sample", "sample_state", "expand_multi_op", "expect", "from_numpy", "get_N", "init", "sample_exact", "seed", "zero_rho", "zero_state",
+    "measurement", "sample", "sample_state", "CompiledCircuit", "Layout", "Options", "apply_twirl", "compile",
+    "get_layers", "get_optimized_layers", "get_stats", "optimize_simple", "expand_multi_op", "expect", "from_numpy", "get_N", "init", "sample_exact", "seed", "zero_rho", "zero_state",
     "Circuit", "Server", "compile_ir", "run", "to_rho", "to_state",
 ]

@@ -287,28 +287,91 @@ class Circuit:
         return idx, nb.value
 
 
-def to_state(ops_or_circuit, N=None, noise=False, seed=0, traj=0, prec="c64"):
-    """to_state (ops.jl:790-795): one trajectory from |0..0>."""
-    c = ops_or_circuit if isinstance(ops_or_circuit, Circuit) else Circuit(ops_or_circuit, N, noise)
+def _uniforms(seed: int, traj: int, start: int, n: int) -> np.ndarray:
+    f = L.lib().bt_uniform
+    return np.array([f(seed, traj, start + d) for d in range(int(n))], dtype=np.float64)
+
+
+def compiled(ops_or_circuit, noise=False, options=None):
+    """An op list compiled as the reference's run / to_state / measure do
+    (compile: swap insertion on the default line layout, then layers), or
+    a CompiledCircuit / device Circuit passed through."""
+    from .compile import CompiledCircuit, Options, compile as _compile
+    if isinstance(ops_or_circuit, (Circuit, CompiledCircuit)):
+        return ops_or_circuit
+    opts = options or Options(noise=noise)
+    return _compile(list(ops_or_circuit), opts)
+
+
+def shot_circuit(cc, seed: int, traj: int, id: int = 0, extra_ops=()):
+    """(device circuit, index of the first apply draw) of one shot of a
+    CompiledCircuit: the twirl rows take the first draws of the shot's stream
+    (seed, traj), in op order."""
+    from .compile import twirl_choice
+    ntw = cc.twirl_count(id)
+    choices = [twirl_choice(u) for u in _uniforms(seed, traj, 0, ntw)]
+    ops = cc.ops_for_shot(id, choices) + list(extra_ops)
+    return Circuit(ops, max(cc.N, 1), cc.options.noise), ntw
+
+
+def run_shot(cc, state: StateVector, seed: int, traj: int, id: int = 0, extra_ops=()):
+    """One shot of a CompiledCircuit on `state` (in place): (outcomes, draws used
+    in total, stats)."""
+    dc, start = shot_circuit(cc, seed, traj, id, extra_ops)
+    if start == 0:
+        outs, st = dc.run_on(state, seed=seed, traj=traj)
+    else:
+        outs, st = dc.run_on(state, uniforms=_uniforms(seed, traj, start, max(dc.max_draws, 1)))
+    return outs, start + st["draws"], st
+
+
+def to_state(ops_or_circuit, N=None, noise=False, seed=0, traj=0, prec="c64", id=0):
+    """to_state (ops.jl:790-795): one trajectory from |0..0> of the compiled
+    circuit's shot ops (stream (seed, traj))."""
+    from .compile import CompiledCircuit
+    c = compiled(ops_or_circuit, noise)
+    if isinstance(c, CompiledCircuit):
+        st = zero_state(max(c.N, N or 0), prec)
+        run_shot(c, st, seed, traj, id)
+        return st
     st = zero_state(c.N, prec)
     c.run_on(st, seed, traj)
     return st
 
 
-def to_rho(ops_or_circuit, N=None, noise=False, prec="c64"):
-    """to_rho (ops.jl:806-844): the exact density matrix."""
-    c = ops_or_circuit if isinstance(ops_or_circuit, Circuit) else Circuit(ops_or_circuit, N, noise)
+def to_rho(ops_or_circuit, N=None, noise=False, prec="c64", id=0, seed=0):
+    """to_rho (ops.jl:806-844): the exact density matrix of the layered ops
+    (ZNE / twirling as to_rho expands them; twirl rows from stream (seed, 0))."""
+    from .compile import CompiledCircuit, twirl_choice
+    c = compiled(ops_or_circuit, noise)
+    if isinstance(c, CompiledCircuit):
+        ch = [twirl_choice(u) for u in _uniforms(seed, 0, 0, c.rho_twirl_count(id))]
+        c = Circuit(c.rho_ops(id, ch), max(c.N, N or 0), c.options.noise)
     rho = zero_rho(c.N, prec)
     c.run_dm(rho)
     return rho
 
 
-def run(ops_or_circuit, number_of_experiment: int = 1, N=None, noise=False, seed=0, prec="c64", batched=True):
-    """run (ops.jl:659-692): mid-circuit outcomes per shot.  Shot t uses the
-    uniform stream (seed, t).  Batched: one persistent launch for all shots."""
+def run(ops_or_circuit, number_of_experiment: int = 1, N=None, noise=False, seed=0, prec="c64", batched=True,
+        id=0, options=None):
+    """run (ops.jl:659-692): mid-circuit outcomes per shot.  An op list is
+    compiled first (swaps, layers), so shots apply vcat(layers...) as the
+    reference does.  Shot t uses the uniform stream (seed, t).  Batched: all
+    shots in one bt_run_batch call (one launch per round); with twirling each
+    shot has its own op list and runs on its own."""
+    from .compile import CompiledCircuit
     if number_of_experiment <= 0:
         raise ValueError("`number_of_experiment` must be positive.")
-    c = ops_or_circuit if isinstance(ops_or_circuit, Circuit) else Circuit(ops_or_circuit, N, noise)
+    c = compiled(ops_or_circuit, noise, options)
+    if isinstance(c, CompiledCircuit):
+        if c.options.twirl:
+            res = []
+            for t in range(number_of_experiment):
+                st = zero_state(max(c.N, N or 0), prec)
+                o, _, _ = run_shot(c, st, seed, t, id)
+                res.append(o)
+            return res
+        c = Circuit(c.ops_for_shot(id), max(c.N, N or 0), c.options.noise)
     if batched:
         outs, _, _ = c.run_batch(number_of_experiment, seed=seed, prec=prec)
         return outs

@@ -171,33 +171,52 @@ def final_basis_ops(N: int, basis: str):
     return [OpQC("RBasis", _R_CHANNEL, q) for q in range(1, N + 1)]
 
 
-def measure_circuit(ops: Sequence[QuantumOps], number_of_experiment: int, N: Optional[int] = None, noise=False,
-                    measurement_basis: str = "Z", readout_noise=False, seed: int = 0, prec="c64",
-                    circuit_name: str = "circuit") -> Measurement:
-    """measure(circuit, number_of_experiment) (ops.jl:599-646).  Without noise
-    and mid-circuit measurements: one state (stream (seed, 0)), rotated to the
-    final basis, sampled number_of_experiment times with the next draws of
-    the same stream.  Otherwise: experiment t is a trajectory on stream
-    (seed, t) sampled once (bt_run_batch_sample), as the reference's
-    per-experiment to_state + sample(state, 1)."""
-    from .circuit import Circuit, get_N
-    if readout_noise is not False and readout_noise is not None:
+def measure_circuit(circuit_or_ops, number_of_experiment: int, N: Optional[int] = None, noise=False,
+                    measurement_basis: Optional[str] = None, readout_noise=False, seed: int = 0, prec="c64",
+                    circuit_name: Optional[str] = None, id: int = 0, twirl: bool = False) -> Measurement:
+    """measure(circuit, number_of_experiment, id) (ops.jl:599-646).  An op list
+    is compiled first (swaps, layers) with Options(noise, measurement_basis,
+    twirl); a compiled circuit brings its own options.  Without noise and
+    mid-circuit measurements: one state (stream (seed, 0)), rotated to the
+    final basis, sampled number_of_experiment times with the next draws of the
+    same stream.  Otherwise: experiment t is a trajectory on stream (seed, t)
+    sampled once with its next draw (bt_run_batch_sample; with twirling, each
+    experiment runs its own op list), as the reference's per-experiment
+    to_state + sample(state, 1)."""
+    from .circuit import Circuit, _uniforms, compiled, run_shot
+    from .compile import CompiledCircuit, Options
+    if isinstance(circuit_or_ops, CompiledCircuit):
+        cc = circuit_or_ops
+    else:
+        cc = compiled(circuit_or_ops, options=Options(
+            circuit_name=circuit_name or "circuit", measurement_basis=measurement_basis or "Z", noise=noise,
+            twirl=twirl, readout_noise=readout_noise))
+    opts = cc.options
+    if opts.readout_noise is not False and opts.readout_noise is not None:
         # the state-vector path calls apply_noise(state, op, ::QuantumChannel),
         # which has no method in the reference (ops.jl:869 vs hilbert.jl:322)
         raise L.BTError("readout noise on the state-vector path is not supported by the reference (ops.jl:869)")
     if number_of_experiment <= 0:
         raise ValueError("number_of_experiment must be positive")
-    n = get_N(ops) if N is None else N
-    has_mid = any(getattr(o, "type", "") == "🔬" for o in ops)
-    full = list(ops) + final_basis_ops(n, measurement_basis)
-    c = Circuit(full, n, noise)
-    if noise is False and not has_mid:
+    n = max(cc.N, N or 0)
+    basis = opts.measurement_basis
+    final = final_basis_ops(n, basis)
+    noiseless = opts.noise is False and cc.stats["mid_measurement_count"] == 0
+    if noiseless:
         st = zero_state(n, prec)
-        _, stats = c.run_on(st, seed=seed, traj=0)
-        used = stats["draws"]
-        u = np.array([L.lib().bt_uniform(seed, 0, used + s) for s in range(number_of_experiment)])
-        idx, nb = sample_with(st, u)
-    else:
+        _, used, _ = run_shot(cc, st, seed, 0, 0, final)
+        idx, nb = sample_with(st, _uniforms(seed, 0, used, number_of_experiment))
+    elif not opts.twirl:
+        c = Circuit(cc.ops_for_shot(id) + final, n, opts.noise)
         idx, nb = c.run_batch_sample(number_of_experiment, seed=seed, prec=prec)
+    else:
+        idx = np.zeros(number_of_experiment, dtype=np.int64)
+        nb = 0
+        for t in range(number_of_experiment):
+            st = zero_state(n, prec)
+            _, used, _ = run_shot(cc, st, seed, t, id, final)
+            i, b = sample_with(st, _uniforms(seed, t, used, 1))
+            idx[t] = i[0]
+            nb += b
     b, p = get_probs_from_sample(idx, n)
-    return _record(b, p, n, measurement_basis.upper(), number_of_experiment, circuit_name, nb)
+    return _record(b, p, n, basis, number_of_experiment, opts.circuit_name, nb)

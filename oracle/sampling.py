@@ -96,10 +96,12 @@ def final_basis_ref_ops(N: int, basis: str):
 
 def measure_circuit(N, ops, shots, noise=None, basis="Z", seed=0, noisy_branch=None):
     """measure(circuit, shots) (ops.jl:599-646) on the oracle with the engine's
-    uniform contract: noiseless -> one state on stream (seed, 0), shots from
-    its next draws; otherwise experiment t = trajectory on stream (seed, t)
-    sampled once with its next draw.  Returns (indices, flags)."""
-    full = list(ops) + final_basis_ref_ops(N, basis)
+    uniform contract: the ops compiled (line-layout swaps, then layer order),
+    then noiseless -> one state on stream (seed, 0), shots from its next
+    draws; otherwise experiment t = trajectory on stream (seed, t) sampled once
+    with its next draw.  Returns (indices, flags)."""
+    import compile_ref as CR
+    full = CR.layered(CR.line_swaps(list(ops))) + final_basis_ref_ops(N, basis)
     has_mid = any(o.kind in (O.KIND_MEASURE, O.KIND_IFOP) for o in ops)
     noisy = noise is not None or has_mid if noisy_branch is None else noisy_branch
     maxd = 8 * (len(full) + 4)

@@ -1,0 +1,219 @@
+"""Circuit compilation: swap insertion, layers, statistics, ZNE / twirl
+expansion (ops.jl:191-445,555-573; noise.jl:136-255; layout.jl:1-110).
+
+CPU part: the engine's host compiler (bluetangle_jl_amd/compile.py) and the
+oracle's restatement (oracle/compile_ref.py) against known answers, against
+each other on random circuits (the oracle computes each op's layer by brute
+force over all earlier ops), and against the mathematics: layering, swap
+insertion and ZNE repetition leave the circuit's unitary unchanged
+(runtests.jl:54-73 style, 1e-12), and every twirl row satisfies
+(P3 x P4) G (P1 x P2) = phase * G.
+GPU part: run / to_rho / measure on compiled circuits against the oracle on
+the same compiled op lists and uniforms."""
+import math
+
+import numpy as np
+import pytest
+
+import compile_ref as CR
+import oracle as O
+from refcirc import brickwork_spec, random_spec, to_bt, to_ref
+
+
+@pytest.fixture(scope="module")
+def cp(bt):
+    import importlib
+    return importlib.import_module(bt.__name__ + ".compile")
+
+
+def names(ops):
+    out = []
+    for o in ops:
+        t = "" if o.target_qubit == -1 else ",%d" % o.target_qubit
+        out.append("%s(%d%s)" % (o.name, o.qubit, t))
+    return out
+
+
+def rand_state(N, rng):
+    v = rng.normal(size=2 ** N) + 1j * rng.normal(size=2 ** N)
+    return v / np.linalg.norm(v)
+
+
+def apply_ref(N, ops, psi):
+    return O.run_sv(N, ops, state=psi)[0]
+
+
+# --- known answers -------------------------------------------------------------
+def test_layers_kat(bt, cp):
+    ops = [bt.Op("H", 1), bt.Op("H", 2), bt.Op("CNOT", 1, 2), bt.Op("X", 3), bt.Op("CNOT", 2, 3), bt.Op("Z", 1)]
+    lay = cp.get_layers(ops)
+    assert [names(l) for l in lay] == [["H(1)", "H(2)", "X(3)"], ["CNOT(1,2)"], ["CNOT(2,3)", "Z(1)"]]
+    st = cp.get_stats(ops)
+    assert st == dict(N=3, len=6, depth=3, cx_count=2, swap_count=0, two_qubit_count=2, mid_measurement_count=0)
+    # a control fences its qubit; an ifOp only its own
+    ops = [bt.Op("X", 1), bt.Op("X", 3, control=1), bt.ifOp("MZ", 2, "X", "I"), bt.Op("Y", 1)]
+    assert [names(l) for l in cp.get_layers(ops)] == [["X(1)", "MZ(2)"], ["X(3)"], ["Y(1)"]]
+
+
+def test_swap_insertion_kat(bt, cp):
+    c = cp.compile([bt.Op("CNOT", 1, 3), bt.Op("CNOT", 4, 1)])
+    assert names(c.ops()) == ["SWAP(1,2)", "CNOT(2,3)", "SWAP(1,2)", "SWAP(4,3)", "SWAP(3,2)", "CNOT(2,1)",
+                              "SWAP(3,2)", "SWAP(4,3)"]
+    assert c.stats_init["len"] == 2 and c.stats["len"] == 8 and c.stats["swap_count"] == 6
+    # swaps are free (noisy_swap = false), the rebuilt op keeps noisy
+    assert [o.noisy for o in c.ops()] == [False, True, False, False, False, True, False, False]
+    with pytest.raises(ValueError):
+        cp.compile([bt.Op("X", 5)], layout=cp.Layout(np.ones((1, 3), dtype=int)))
+
+
+def test_swap_drops_control(bt, cp):
+    """layout.jl:86 rebuilds Op(op.name, a, target; noisy): a control on a
+    non-adjacent 2-qubit op is lost, in the oracle's restatement too."""
+    c = cp.compile([bt.Op("CCX", 1, 4, 3)])  # control 4 on CX(1, 3)
+    assert names(c.ops()) == ["SWAP(1,2)", "CX(2,3)", "SWAP(1,2)"] and all(o.control == -2 for o in c.ops())
+    ref = CR.line_swaps([O.ref_gate("CX", 1, 3, 4)])
+    assert [(o.name, o.qubit, o.target, o.control) for o in ref] == [("SWAP", 1, 2, -2), ("CX", 2, 3, -2),
+                                                                       ("SWAP", 1, 2, -2)]
+
+
+def test_layout_grid_kat(cp):
+    L = cp.Layout(np.array([[1, 1], [1, 1]]))
+    # numbered column-major: (0,0)=1, (1,0)=2, (0,1)=3, (1,1)=4
+    assert L.neighbors == {1: [2, 3], 2: [1, 4], 3: [1, 4], 4: [2, 3]}
+    assert L.connectivity == 2.0 and L.N == 4
+    L = cp.Layout(np.array([[1, 0, 1], [1, 1, 1]]))
+    assert L.N == 5 and L.shortest_path(1, 4) == [1, 2, 3, 5, 4]
+
+
+def test_optimize_simple_kat(bt, cp):
+    ops = [bt.Op("X", 1), bt.Op("X", 1), bt.Op("H", 2), bt.Op("CNOT", 2, 3), bt.Op("CNOT", 2, 3), bt.Op("Z", 3)]
+    lay = cp.get_optimized_layers(ops)
+    assert [names(l) for l in lay] == [["H(2)", "Z(3)"]]
+    ops = [bt.Op("S", 1)] * 4 + [bt.Op("S", 2)] * 3
+    assert [names(l) for l in cp.get_optimized_layers(ops)] == [["S(2)"], ["S(2)"], ["S(2)"]]
+
+
+# --- engine compiler vs oracle restatement -------------------------------------------
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_compile_matches_oracle(bt, cp, seed):
+    rng = np.random.default_rng(seed)
+    N = 6
+    spec = random_spec(N, 40, rng, kinds=("gate1", "gate2", "meas", "chan1", "ifop", "reset"), nonadjacent=True,
+                       controls=True)
+    c = cp.compile(to_bt(spec, bt))
+    ref = CR.layered(CR.line_swaps(to_ref(spec)))
+    assert len(c.ops()) == len(ref)
+    for a, b in zip(c.ops(), ref):
+        assert (a.qubit, a.target_qubit) == (b.qubit, b.target) and a.name.upper() == b.name.upper()
+    assert c.stats["depth"] == CR.depth(CR.line_swaps(to_ref(spec)))
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_compile_preserves_unitary(bt, cp, seed):
+    """runtests.jl:54-73: the compiled (swapped, layered) circuit applied to a
+    random state equals the ops applied in order."""
+    rng = np.random.default_rng(seed)
+    N = 6
+    # (doubly controlled ops are left out: swap insertion rebuilds a
+    # non-adjacent op from its name and drops its control, layout.jl:86,
+    # which changes the circuit; test_swap_drops_control pins that)
+    spec = random_spec(N, 50, rng, kinds=("gate1", "gate2"), nonadjacent=True, controls=True)
+    ref = to_ref(spec)
+    psi = rand_state(N, rng)
+    a = apply_ref(N, ref, psi)
+    b = apply_ref(N, CR.layered(CR.line_swaps(ref)), psi)
+    assert np.abs(a - b).max() < 1e-12
+    # ZNE: 2 id + 1 copies of every CNOT leave it unchanged
+    c = apply_ref(N, CR.shot_ops(CR.layered(CR.line_swaps(ref)), id=2), psi)
+    assert np.abs(a - c).max() < 1e-12
+
+
+def test_twirl_rows(bt, cp):
+    """Every row leaves its gate unchanged up to a phase; the engine's rows are
+    the oracle's (CNOT) and twirled shots keep the unitary."""
+    P = {k: O.GATE[k] for k in "IXYZ"}
+    SW = O.GATE["SWAP"]
+    mats = {"cnot": O.GATE["CNOT"], "cnot_rev": SW @ O.GATE["CNOT"] @ SW, "ecr": O.GATE["ECR"],
+            "ecr_rev": SW @ O.GATE["ECR"] @ SW}
+    for key, rows in cp._TWIRL.items():
+        assert len(set(rows)) == 16
+        G = mats[key]
+        for r in rows:
+            U = np.kron(P[r[2]], P[r[3]]) @ G @ np.kron(P[r[0]], P[r[1]])
+            k = np.argmax(np.abs(G))
+            assert np.allclose(U, U.flat[k] / G.flat[k] * G, atol=1e-14), (key, r)
+    assert ["".join(r) for r in cp._TWIRL["cnot"]] == CR.TWIRL_CNOT
+    assert ["".join(r) for r in cp._TWIRL["cnot_rev"]] == CR.TWIRL_CNOT_REV
+    rng = np.random.default_rng(9)
+    N = 5
+    ref = CR.layered(to_ref(brickwork_spec(N, 3, seed=4)))
+    ncx = sum(1 for o in ref if o.name.upper() in ("CNOT", "CX"))
+    ch = list(rng.integers(0, 16, size=3 * ncx))
+    psi = rand_state(N, rng)
+    a = apply_ref(N, ref, psi)
+    b = apply_ref(N, CR.shot_ops(ref, id=1, twirl=True, choices=ch), psi)
+    assert abs(abs(np.vdot(a, b)) - 1) < 1e-12
+
+
+def test_shot_expansion_matches_oracle(bt, cp):
+    N = 5
+    spec = brickwork_spec(N, 4, seed=7)
+    c = cp.compile(to_bt(spec, bt), cp.Options(twirl=True))
+    ref = CR.layered(CR.line_swaps(to_ref(spec)))
+    n = c.twirl_count(1)
+    ch = [int(x) for x in np.random.default_rng(1).integers(0, 16, size=n)]
+    a = c.ops_for_shot(1, ch)
+    b = CR.shot_ops(ref, id=1, twirl=True, choices=ch)
+    assert [(o.name.upper(), o.qubit, o.target_qubit) for o in a] == [(o.name.upper(), o.qubit, o.target) for o in b]
+    assert cp.twirl_choice(0.0) == 0 and cp.twirl_choice(0.999999) == 15 and cp.twirl_choice(0.5) == 8
+
+
+# --- engine (GPU) ------------------------------------------------------------------
+@pytest.mark.gpu
+def test_gpu_run_compiled_vs_oracle(bt):
+    """run(ops, shots; noise) applies vcat(layers...) (ops.jl:659-692): outcomes
+    per shot equal the oracle's on the compiled op list and the same streams."""
+    N = 6
+    rng = np.random.default_rng(12)
+    spec = random_spec(N, 30, rng, kinds=("gate1", "gate2", "meas", "chan1"), nonadjacent=True, controls=False)
+    nm = bt.NoiseModel(bt.Noise1("depolarizing", 0.05), bt.Noise2("depolarizing", 0.1))
+    nm_ref = O.ref_noise("depolarizing", 0.05, "depolarizing", 0.1)
+    shots = 64
+    got = bt.run(to_bt(spec, bt), shots, noise=nm, seed=3, prec="c128")
+    ref_ops = CR.layered(CR.line_swaps(to_ref(spec)))
+    want, _, _ = O.run_traj(N, ref_ops, nm_ref, 3, 0, shots, 8 * len(ref_ops) + 8)
+    assert got == want
+
+
+@pytest.mark.gpu
+def test_gpu_run_twirl_zne_vs_oracle(bt):
+    N = 5
+    spec = brickwork_spec(N, 3, seed=2, mz_prob=0.3)
+    nm = bt.NoiseModel(bt.Noise1("depolarizing", 0.05), bt.Noise2("depolarizing", 0.1))
+    nm_ref = O.ref_noise("depolarizing", 0.05, "depolarizing", 0.1)
+    c = bt.compile(to_bt(spec, bt), bt.Options(noise=nm, twirl=True))
+    shots = 12
+    got = bt.run(c, shots, seed=5, prec="c128", id=1)
+    ref = CR.layered(CR.line_swaps(to_ref(spec)))
+    ntw = c.twirl_count(1)
+    assert ntw == 3 * sum(1 for o in ref if o.name.upper() == "CNOT")
+    want = []
+    for t in range(shots):
+        u = O.uniforms(5, t, ntw + 4000)
+        ch = [min(int(x * 16), 15) for x in u[:ntw]]
+        _, outs, _, _, _ = O.run_sv(N, CR.shot_ops(ref, id=1, twirl=True, choices=ch), nm_ref, u=u[ntw:])
+        want.append(outs)
+    assert got == want
+
+
+@pytest.mark.gpu
+def test_gpu_to_rho_zne(bt):
+    N = 4
+    spec = brickwork_spec(N, 3, seed=8, ad_gamma=0.05)
+    nm = bt.NoiseModel(bt.Noise1("depolarizing", 0.05), bt.Noise2("depolarizing", 0.1))
+    nm_ref = O.ref_noise("depolarizing", 0.05, "depolarizing", 0.1)
+    c = bt.compile(to_bt(spec, bt), bt.Options(noise=nm))
+    for id in (0, 1):
+        rho = bt.to_rho(c, id=id, prec="c128").to_numpy()
+        ref = O.run_dm(N, CR.shot_ops(CR.layered(CR.line_swaps(to_ref(spec))), id=id), nm_ref)
+        assert np.abs(rho - ref).max() < 1e-12
