@@ -14,6 +14,7 @@ from .core import (DensityMatrix, DeviceState, StateVector, UniformStream, apply
 from .circuit import Circuit, Server, compile_ir, run, to_rho, to_state
 from .observables import correlation, expand_multi_op, expect
 from . import measurement
+from .qasm import from_qasm, to_qasm
 from .compile import (CompiledCircuit, Layout, Options, apply_twirl, compile, get_layers, get_optimized_layers,
                       get_stats, optimize_simple)
 from .measurement import (Measurement, get_probs_from_sample, measure, measure_circuit, sample, sample_state)
@@ -24,6 +25,6 @@ __all__ = [
     "ifOp", "DensityMatrix", "DeviceState", "StateVector", "UniformStream", "apply", "apply_noise",
     "current_stream", "correlation", "Measurement", "get_probs_from_sample", "measure", "measure_circuit",
     "measurement", "sample", "sample_state", "CompiledCircuit", "Layout", "Options", "apply_twirl", "compile",
-    "get_layers", "get_optimized_layers", "get_stats", "optimize_simple", "expand_multi_op", "expect", "from_numpy", "get_N", "init", "sample_exact", "seed", "zero_rho", "zero_state",
+    "get_layers", "get_optimized_layers", "get_stats", "optimize_simple", "from_qasm", "to_qasm", "expand_multi_op", "expect", "from_numpy", "get_N", "init", "sample_exact", "seed", "zero_rho", "zero_state",
     "Circuit", "Server", "compile_ir", "run", "to_rho", "to_state",
 ]

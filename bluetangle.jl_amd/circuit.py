@@ -299,6 +299,9 @@ def compiled(ops_or_circuit, noise=False, options=None):
     from .compile import CompiledCircuit, Options, compile as _compile
     if isinstance(ops_or_circuit, (Circuit, CompiledCircuit)):
         return ops_or_circuit
+    if isinstance(ops_or_circuit, str):  # OpenQASM text or file (run(qasm_or_path), ops.jl:720-727)
+        from .qasm import from_qasm, qasm_source
+        ops_or_circuit = from_qasm(qasm_source(ops_or_circuit))
     opts = options or Options(noise=noise)
     return _compile(list(ops_or_circuit), opts)
 
@@ -354,7 +357,8 @@ def to_rho(ops_or_circuit, N=None, noise=False, prec="c64", id=0, seed=0):
 
 def run(ops_or_circuit, number_of_experiment: int = 1, N=None, noise=False, seed=0, prec="c64", batched=True,
         id=0, options=None):
-    """run (ops.jl:659-692): mid-circuit outcomes per shot.  An op list is
+    """run (ops.jl:659-692, and run(qasm_or_path) ops.jl:720-727):
+    mid-circuit outcomes per shot.  An op list (or OpenQASM text / file) is
     compiled first (swaps, layers), so shots apply vcat(layers...) as the
     reference does.  Shot t uses the uniform stream (seed, t).  Batched: all
     shots in one bt_run_batch call (one launch per round); with twirling each
@@ -433,6 +437,12 @@ class Server:
                                          ctypes.byref(rid)))
         self._circ[rid.value] = circuit
         return rid.value
+
+    def submit_qasm(self, qasm: str, kind="sv", prec="c64", seed=0, traj=0, noise=False) -> int:
+        """A request in OpenQASM 2 (text or file): imported (qasm.jl:275-421),
+        compiled to layer order, lowered with `noise`, and submitted."""
+        cc = compiled(qasm, noise)
+        return self.submit(Circuit(cc.ops(), cc.N, noise), kind, prec, seed, traj)
 
     def poll(self, wait=False, cap=256):
         """Finished requests: list of dicts (id, status, nout, latency_ms, queue_ms)."""
