@@ -408,6 +408,19 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
     moff += D2 * D2;
   }
   g->nmats = moff;
+  // the input scale rides on the first operator that every amplitude of
+  // every tile goes through (no out-of-tile control), sparing the per-tile
+  // multiply of each register
+  for (int e = 0; e < plan.nops && g->scale != 1.0; e++) {
+    const ROp& r = g->ops[e];
+    if (r.gctrl) continue;
+    const int D2 = 1 << r.k;
+    for (int i = 0; i < D2 * D2; i++) {
+      mats[r.moff + i].x = (R)((double)mats[r.moff + i].x * g->scale);
+      mats[r.moff + i].y = (R)((double)mats[r.moff + i].y * g->scale);
+    }
+    g->scale = 1.0;
+  }
   for (int p = 0; p < nph; p++) make_slots<R>(g->ph[p].rb, T, &g->lay[p]);
   make_slots<R>(g->io_in, T, &g->lay[kLayIn]);
   make_slots<R>(g->io_out, T, &g->lay[kLayOut]);
@@ -746,6 +759,44 @@ __host__ __device__ __forceinline__ void opaque_io(ThreadIO<O>& io) {
   opaque(io.gthr_out);
 }
 
+// State loads and stores on the device go through the global address space
+// (global_load / global_store, counted by vmcnt alone).  Through a generic
+// pointer they compile to flat_* accesses, which also count against lgkmcnt,
+// so every LDS or scalar-load wait in the pass would also wait for the
+// in-flight prefetch of the next tile and the stores of the last one.
+template <typename R>
+__host__ __device__ __forceinline__ C<R> gload(const C<R>* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef R v2 __attribute__((ext_vector_type(2)));  // a builtin vector: no struct copy through a generic pointer
+  const v2 x = *(const __attribute__((address_space(1))) v2*)p;
+  return cmake<C<R>>(x.x, x.y);
+#else
+  return *p;
+#endif
+}
+template <typename R>
+__host__ __device__ __forceinline__ void gstore(C<R>* p, const C<R>& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  v2 x;
+  x.x = v.x;
+  x.y = v.y;
+  *(__attribute__((address_space(1))) v2*)p = x;
+#else
+  *p = v;
+#endif
+}
+
+// A flag that is the same in every lane (a pass-geometry field), moved to a
+// scalar register so branches on it are scalar branches.
+__host__ __device__ __forceinline__ bool uflag(bool x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_readfirstlane((int)x) != 0;
+#else
+  return x;
+#endif
+}
+
 // global index of tile t's first amplitude
 __host__ __device__ __forceinline__ uint64_t tile_base(const PassGeom& g, uint64_t t) {
   uint64_t base = 0;
@@ -772,7 +823,7 @@ __host__ __device__ __forceinline__ void fetch_regs(C<R>* raw, const C<R>* __res
 #pragma unroll
     for (int j = 0; j < kRB; j++)
       if ((r >> j) & 1) off |= io.gin[j];
-    raw[r] = tp[off];
+    raw[r] = gload<R>(tp + off);
   }
 }
 
@@ -783,6 +834,14 @@ template <typename R, typename O>
 __host__ __device__ __forceinline__ void take_regs(C<R>* v, const C<R>* raw, const ThreadIO<O>& io, uint64_t base,
                                                    R scale, int fresh) {
   constexpr int NR = 1 << TileCfg<R>::RB;
+  if (!fresh && io.active && io.nvalid_mask == 0) {  // every register real: no masks
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      v[r].x = raw[r].x * scale;
+      v[r].y = raw[r].y * scale;
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     const bool ok = io.active && !(r & io.nvalid_mask);
@@ -806,8 +865,28 @@ __host__ __device__ __forceinline__ void store_regs(const C<R>* v, C<R>* __restr
                                                     const ThreadIO<O>& io, uint64_t base, double* acc) {
   constexpr int NR = 1 << TileCfg<R>::RB;
   C<R>* __restrict__ tp = state + base;
-  const bool slow_hist = g.red == bti::RED_HIST && !g.hist_fast;
+  const bool slow_hist = uflag(g.red == bti::RED_HIST && !g.hist_fast);
+  const bool write = uflag(g.write), need_norm = uflag(g.need_norm);
   double s = 0.0;
+  if (io.active && io.nvalid_mask == 0 && !slow_hist) {  // every register real: no masks
+    if (write) {
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        O off = io.gthr_out;
+#pragma unroll
+        for (int j = 0; j < kRB; j++)
+          if ((r >> j) & 1) off |= io.gout[j];
+        gstore<R>(tp + off, v[r]);
+      }
+    }
+    if (need_norm) {
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const double w = (double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y;
+        s += w;
+      }
+    }
+  } else {
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     O off = io.gthr_out;
@@ -815,7 +894,7 @@ __host__ __device__ __forceinline__ void store_regs(const C<R>* v, C<R>* __restr
     for (int j = 0; j < kRB; j++)
       if ((r >> j) & 1) off |= io.gout[j];
     if (io.active && !(r & io.nvalid_mask)) {
-      if (g.write) tp[off] = v[r];
+      if (g.write) gstore<R>(tp + off, v[r]);
       if (g.need_norm) {
         const double w = (double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y;
         s += w;
@@ -828,6 +907,7 @@ __host__ __device__ __forceinline__ void store_regs(const C<R>* v, C<R>* __restr
         }
       }
     }
+  }
   }
   acc[0] += s;
   if (g.red == bti::RED_HIST && g.hist_fast) {  // one bin for all of this thread's amplitudes
@@ -900,13 +980,17 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
   ThreadIO<O> io;
   thread_io<R, O>(g, threadIdx.x, io);
   C<R> v[NR], nx[NR];
-  if (!fresh && t0 < tend) fetch_regs<R, O>(nx, state, io, uniform_u64(tile_base(g, t0)));
+  // tile t's base is t's bits deposited on the outside bits M (ascending), so
+  // the next tile's base is a masked add: ((base | ~M) + deposit(tstep)) & M
+  const uint64_t M = uniform_u64(tile_base(g, ~0ull)), S = uniform_u64(tile_base(g, tstep));
+  uint64_t base = uniform_u64(tile_base(g, t0));
+  if (!fresh && t0 < tend) fetch_regs<R, O>(nx, state, io, base);
   for (uint64_t t = t0; t < tend; t += tstep) {
     opaque_io(io);
-    const uint64_t base = uniform_u64(tile_base(g, t));
+    const uint64_t nbase = ((base | ~M) + S) & M;
     take_regs<R, O>(v, nx, io, base, scale, fresh);
     // the next tile's loads fly while this one computes (tiles are disjoint)
-    if (!fresh && t + tstep < tend) fetch_regs<R, O>(nx, state, io, uniform_u64(tile_base(g, t + tstep)));
+    if (!fresh && t + tstep < tend) fetch_regs<R, O>(nx, state, io, nbase);
     int8_t cur[kRB];
 #pragma unroll
     for (int j = 0; j < kRB; j++) cur[j] = g.io_in[j];
@@ -967,6 +1051,7 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
     } else if (g.need_norm) {
       wave_accumulate<1>(acc, wred);
     }
+    base = nbase;
   }
   __syncthreads();
   if (threadIdx.x < kAcc) {

@@ -26,6 +26,11 @@ using cd = std::complex<double>;
 
 static std::string g_msg;
 
+static int emu_fail(const char* m) {
+  g_msg = m;
+  return BT_ERR_STATE;
+}
+
 static void apply_op(std::vector<cd>& s, int nbits, const bti::QOp& op) {
   const int k = op.k, D = 1 << k;
   uint64_t tmask = 0;
@@ -108,8 +113,15 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
   for (int tid = 0; tid < NT; tid++) thread_io<R, O>(g, tid, io[tid]);
   double sums[kAcc];
   for (int j = 0; j < kAcc; j++) sums[j] = 0.0;
+  // the device advances tile bases by a masked add of the deposited step;
+  // check it against the direct deposit for a few steps
+  const uint64_t M = tile_base(g, ~0ull);
+  uint64_t base = tile_base(g, 0);
   for (uint64_t t = 0; t < ntiles; t++) {
-    const uint64_t base = tile_base(g, t);
+    if (base != tile_base(g, t)) return emu_fail("masked-add tile base mismatch");
+    for (uint64_t step : {3ull, 7ull, 64ull})
+      if (t + step < ntiles && (((base | ~M) + tile_base(g, step)) & M) != tile_base(g, t + step))
+        return emu_fail("masked-add tile base mismatch (step)");
     for (int tid = 0; tid < NT; tid++) {
       std::array<C<R>, NR> raw{};
       if (!fresh) fetch_regs<R, O>(raw.data(), st.data(), io[tid], base);
@@ -168,6 +180,7 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
       else if (g.need_norm)
         sums[0] += acc[0];
     }
+    base = ((base | ~M) + tile_base(g, 1)) & M;
   }
   to_result(plan, g.red_swap, sums, res);
   return BT_OK;
