@@ -81,9 +81,26 @@ struct LSlots {
 
 struct RPhase {
   int8_t rb[kRB];  // local tile bit held at register position j (-1: phantom)
-  int8_t pad[3];
+  int8_t tmpl;     // > 0: the phase's operators match kPhaseTemplates[tmpl]
+  int8_t pad[2];
   int32_t op0, nops;
 };
+
+// Register-mask sequences that start a phase and that the device applies as
+// one straight-line block (no per-operator dispatch, whose branch chain
+// copies the 2^RB live registers at every level); the phase's remaining
+// operators follow one by one.  These are the openings layered brickworks
+// produce most (two disjoint register pairs).  Phases with out-of-tile
+// controls or LDS operators never match.
+struct PhaseTemplate {
+  int8_t n;
+  int8_t mask[4];
+};
+constexpr PhaseTemplate kPhaseTemplates[] = {
+    {0, {0, 0, 0, 0}},
+    {2, {0x0c, 0x03, 0, 0}},
+};
+constexpr int kNumPhaseTemplates = (int)(sizeof(kPhaseTemplates) / sizeof(kPhaseTemplates[0]));
 
 constexpr int kLayIn = kMaxPh, kLayOut = kMaxPh + 1;  // slot tables of the load / store layouts
 
@@ -408,6 +425,27 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
     moff += D2 * D2;
   }
   g->nmats = moff;
+  // straight-line phase templates
+  for (int p = 0; p < nph; p++) {
+    RPhase& ph = g->ph[p];
+    ph.tmpl = 0;
+    bool plain = ph.nops > 0;
+    for (int e = ph.op0; e < ph.op0 + ph.nops && plain; e++)
+      plain = g->ops[e].k <= 2 && g->ops[e].gctrl == 0;
+    if (!plain) continue;
+    for (int t = 1; t < kNumPhaseTemplates; t++) {
+      const PhaseTemplate& pt = kPhaseTemplates[t];
+      bool ok = pt.n <= ph.nops;  // a template covers the phase's first pt.n operators
+      for (int i = 0; i < pt.n && ok; i++) {
+        ok = g->ops[ph.op0 + i].mask == pt.mask[i];
+        if (TileCfg<R>::RB < 5) ok = ok && pt.mask[i] < 16;
+      }
+      if (ok) {
+        ph.tmpl = (int8_t)t;
+        break;
+      }
+    }
+  }
   // the input scale rides on the first operator that every amplitude of
   // every tile goes through (no out-of-tile control), sparing the per-tile
   // multiply of each register
@@ -502,6 +540,30 @@ __host__ __device__ __forceinline__ void apply_mask(int mask, C<R>* v, const C<R
 #undef BT_CASE5
     default: break;
   }
+}
+
+// A matched phase template: the operators' register applications in order,
+// straight-line (the masks are compile-time constants of each case).
+// Keeps the next operator's matrix loads from being hoisted above this
+// operator (all of a template's matrices live at once would spill).
+__host__ __device__ __forceinline__ void op_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" ::: "memory");
+#endif
+}
+
+// A matched phase template: the operators' register applications in order,
+// straight-line (the masks are compile-time constants of each case).
+template <typename R>
+__host__ __device__ __forceinline__ void apply_template(int tm, C<R>* v, const C<R>* mats, const ROp* ops) {
+#define BT_A(M, I)                              \
+  apply_reg<R, M>(v, mats + ops[I].moff); \
+  op_fence();
+  if (tm == 1) {
+    BT_A(0x0c, 0)
+    BT_A(0x03, 1)
+  }
+#undef BT_A
 }
 
 // reduced rho over register positions P0 < P1 (K = 1 or 2), packed as
@@ -925,6 +987,8 @@ __host__ __device__ __forceinline__ int rho_reg_mask(const PassGeom& g) {
 
 #if defined(__HIPCC__)
 // A value the whole workgroup holds in common, moved to scalar registers.
+__device__ __forceinline__ int uniform_i32(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
@@ -1014,7 +1078,9 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
       }
       if (p == g.nph) break;
       const int op0 = g.ph[p].op0, op1 = op0 + g.ph[p].nops;
-      for (int o = op0; o < op1; o++) {
+      const int tm = uniform_i32(g.ph[p].tmpl);
+      if (tm > 0) apply_template<R>(tm, v, mats, g.ops + op0);
+      for (int o = op0 + (tm > 0 ? 2 : 0); o < op1; o++) {  // every template covers two operators
         const ROp& op = g.ops[o];
         if ((base & op.gctrl) != op.gctrl) continue;
         if (op.k <= 2) {

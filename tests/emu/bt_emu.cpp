@@ -68,6 +68,7 @@ static int g_exec = 0;
 static int64_t g_gstat[9];
 
 static FILE* g_geom_dump = nullptr;
+static int64_t g_tmpl_phases = 0;  // phases run through a straight-line template
 
 extern "C" void emu_set_exec(int mode) { g_exec = mode; }
 // Append every register-tile pass geometry (PassGeom, then nmats complex
@@ -77,6 +78,12 @@ extern "C" int emu_dump_geometry(const char* path) {
   g_geom_dump = path ? fopen(path, "wb") : nullptr;
   return path && !g_geom_dump ? -1 : 0;
 }
+extern "C" int64_t emu_template_phases(int reset) {
+  const int64_t v = g_tmpl_phases;
+  if (reset) g_tmpl_phases = 0;
+  return v;
+}
+
 extern "C" void emu_geom_stats(int64_t* out, int reset) {
   for (int i = 0; i < 9; i++) out[i] = g_gstat[i];
   if (reset) memset(g_gstat, 0, sizeof g_gstat);
@@ -146,7 +153,11 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
       }
       if (p == g.nph) break;
       const int op0 = g.ph[p].op0, op1 = op0 + g.ph[p].nops;
-      for (int o = op0; o < op1; o++) {
+      if (g.ph[p].tmpl > 0) {  // as the device: the matched template, straight-line
+        for (int tid = 0; tid < NT; tid++) apply_template<R>(g.ph[p].tmpl, v[tid].data(), mats.data(), g.ops + op0);
+        g_tmpl_phases++;
+      }
+      for (int o = op0 + (g.ph[p].tmpl > 0 ? kPhaseTemplates[g.ph[p].tmpl].n : 0); o < op1; o++) {
         const ROp& op = g.ops[o];
         if ((base & op.gctrl) != op.gctrl) continue;
         if (op.k <= 2) {

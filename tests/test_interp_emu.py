@@ -175,12 +175,16 @@ def test_register_tile_pass_brickwork(emu, bt, mode):
     r, routs, used, bnd = oracle_run(spec, N, nm_args, 4, tol=tol)
     assert not bnd
     emu.emu_set_exec(mode)
+    emu.emu_template_phases.restype = ctypes.c_int64
+    emu.emu_template_phases(1)
     try:
         s, outs, stats = emu_run(emu, bt, spec, N, bt.NoiseModel(*nm_args), 4, T=12 if mode == 1 else 13, cmin=1, tol=tol)
     finally:
         emu.emu_set_exec(0)
     assert outs == routs
     assert np.abs(s - r).max() < (1e-11 if mode == 1 else 1e-5)
+    # the straight-line phase templates were exercised
+    assert emu.emu_template_phases(1) > 0
 
 
 def _apply_block_dense(state, N, k, bits, m):
