@@ -527,6 +527,18 @@ __host__ __device__ __forceinline__ void apply_reg(C<R>* v, const C<R>* mats) {
 
 template <typename R>
 __host__ __device__ __forceinline__ void apply_mask(int mask, C<R>* v, const C<R>* m) {
+  // adjacent register pairs are the common case (brickwork pairs after the
+  // phase planner): test them first, every level of the dispatch costs moves
+  if (__builtin_expect(mask == 0x06, 1)) {
+    apply_reg<R, 0x06>(v, m);
+    return;
+  }
+  if constexpr (TileCfg<R>::RB == 5) {
+    if (__builtin_expect(mask == 0x18, 1)) {
+      apply_reg<R, 0x18>(v, m);
+      return;
+    }
+  }
   switch (mask) {
 #define BT_CASE(M) \
   case M: apply_reg<R, M>(v, m); break;
