@@ -330,11 +330,15 @@ __global__ void k_gather(const C<R>* s, uint64_t len, const uint64_t* idx, uint6
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// Workgroups of a reduction with M partial values each: up to 4096 (eight
-// resident per CU), within the partial buffer of kRedBlocks * kRedMax values.
-static uint64_t red_cap(int M) {
+// Workgroups of a reduction with M partial values each over `work` items:
+// 4096 (eight resident per CU) for large states, where more bytes in flight
+// pay; kRedBlocks below 2^26 items, where the launch and the final sum of
+// the partials dominate; always within the partial buffer of
+// kRedBlocks * kRedMax values.
+static uint64_t red_cap(int M, uint64_t work) {
   const uint64_t c = (uint64_t)kRedBlocks * kRedMax / (uint64_t)M;
-  return c < 4096 ? c : 4096;
+  const uint64_t want = work >= (1ull << 26) ? 4096 : kRedBlocks;
+  return c < want ? c : want;
 }
 
 static unsigned grid_for(uint64_t work, unsigned block = 256, uint64_t cap = (1ull << 20)) {
@@ -427,7 +431,7 @@ static int reduce_t(bt_state_s* s, const Placement& p) {
   a.nfixed = p.nfixed;
   for (int i = 0; i < p.nfixed; i++) a.fixed[i] = p.fixed[i];
   a.part = s->d_part;
-  const unsigned blocks = grid_for(a.ngroups, 256, red_cap(M));
+  const unsigned blocks = grid_for(a.ngroups, 256, red_cap(M, a.ngroups));
   hipLaunchKernelGGL((k_reduce<R, K>), dim3(blocks), dim3(256), 0, s->stream, a);
   BT_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finalize, dim3(M), dim3(256), 0, s->stream, (const double*)s->d_part, (int)blocks, M, s->d_out);
@@ -439,7 +443,7 @@ static int reduce_t(bt_state_s* s, const Placement& p) {
 
 int launch_inner(bt_state_s* a, bt_state_s* b, double* re, double* im) {
   const uint64_t n = a->len;
-  const unsigned blocks = grid_for(n, 256, red_cap(2));
+  const unsigned blocks = grid_for(n, 256, red_cap(2, n));
   const bool fa = a->prec == BT_C64, fb = b->prec == BT_C64;
   if (fa && fb)
     hipLaunchKernelGGL((k_inner<float, float>), dim3(blocks), dim3(256), 0, a->stream, (const C<float>*)a->d,
@@ -610,7 +614,7 @@ int launch_expect(bt_state_s* s, int k, const double* mat, const int* bits, uint
     for (int i = 0; i < 2 * D * D; i++) a.g[i] = mat[i];
   }
   a.ngroups = 1ull << (n - a.nfixed);
-  const unsigned blocks = grid_for(a.ngroups, 256, red_cap(2));
+  const unsigned blocks = grid_for(a.ngroups, 256, red_cap(2, a.ngroups));
   const bool f = s->prec == BT_C64;
   switch (k) {
     case 0: f ? expect_t<float, 0>(s, a, blocks) : expect_t<double, 0>(s, a, blocks); break;
