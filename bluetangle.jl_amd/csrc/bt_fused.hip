@@ -6,10 +6,12 @@
 // and x = (Re, Im) interleaved.  k_fused5 streams the state in tiles of
 // 2^12 amplitudes (the 5 block bits + the 7 lowest other bits): a coalesced
 // load into LDS, then every wavefront multiplies 32 of the tile's 128 groups
-// with v_mfma_f32_32x32x2f32 (A resident in registers, the groups as the 32
-// columns: lanes 0-31 supply the real and lanes 32-63 the imaginary part of
-// each component), writes its results back over its own groups, and the tile
-// is stored coalesced.  Per amplitude: 128 real MACs = 16 flop/byte of HBM.
+// with v_mfma_f32_32x32x2f32 (the groups as the 32 columns: lanes 0-31
+// supply the real and lanes 32-63 the imaginary part of each component; A is
+// staged once per workgroup in LDS in lane order and read per step), writes
+// its results back over its own groups, and the tile is stored coalesced.
+// 168 VGPRs and 48 KiB of LDS: three workgroups per CU.  Per amplitude: 128
+// real MACs = 16 flop/byte of HBM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,16 +44,12 @@ struct F5Args {
 // LDS slot of local index l: XOR-swizzle of the low 5 bits (8-byte elements)
 __device__ __forceinline__ uint32_t f5_swz(uint32_t l) { return l ^ (((l >> 5) ^ (l >> 10)) & 31u); }
 
-__global__ void __launch_bounds__(256, 2) k_fused5(F5Args a, const float* __restrict__ Alane) {
+__global__ void __launch_bounds__(256, 3) k_fused5(F5Args a, const float* __restrict__ Alane) {
   __shared__ float2 lds[kF5N];
+  __shared__ float la[64 * 64];  // A in lane order: la[(rb * 32 + i) * 64 + lane]
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // A operand of instruction i, row block rb: lane holds A[32 rb + lane % 32][2 i + lane / 32]
-  float a0[32], a1[32];
 #pragma unroll
-  for (int i = 0; i < 32; i++) {
-    a0[i] = Alane[i * 64 + lane];
-    a1[i] = Alane[(32 + i) * 64 + lane];
-  }
+  for (int i = 0; i < 16; i++) la[tid + 256 * i] = Alane[tid + 256 * i];
   // this lane's group: g = 32 * wave + lane % 32 -> local base
   const uint32_t grp = 32 * wave + (lane & 31);
   uint32_t gb = 0;
@@ -99,22 +97,20 @@ __global__ void __launch_bounds__(256, 2) k_fused5(F5Args a, const float* __rest
     // the next tile's loads fly during this tile's MFMA work (tiles are disjoint)
     if (t + gridDim.x < a.ntiles) fetch(tile_base(t + gridDim.x));
     __syncthreads();
-    float bv[32];
-#pragma unroll
-    for (int c = 0; c < 32; c++) {
-      uint32_t sc = 0;
-#pragma unroll
-      for (int i = 0; i < 5; i++)
-        if ((c >> (4 - i)) & 1) sc ^= cb[i];
-      const float2 e = lds[s0 ^ sc];
-      bv[c] = hi_half ? e.y : e.x;
-    }
     floatx16 d0 = {}, d1 = {};
 #pragma unroll
     for (int i = 0; i < 32; i++) {
-      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[i], bv[i], d0, 0, 0, 0);
-      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[i], bv[i], d1, 0, 0, 0);
+      // B: component i of this lane's group (real part in lanes 0-31, imaginary in 32-63)
+      uint32_t sc = 0;
+#pragma unroll
+      for (int k = 0; k < 5; k++)
+        if ((i >> (4 - k)) & 1) sc ^= cb[k];
+      const float2 e = lds[s0 ^ sc];
+      const float bv = hi_half ? e.y : e.x;
+      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(la[i * 64 + lane], bv, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(la[(32 + i) * 64 + lane], bv, d1, 0, 0, 0);
     }
+    // each wavefront reads and then overwrites only its own 32 groups: no barrier here
     // row m of block rb = 8 (j / 4) + 4 (lane / 32) + j % 4: (re, im) of output 16 rb + m / 2
 #pragma unroll
     for (int j = 0; j < 16; j += 2) {
