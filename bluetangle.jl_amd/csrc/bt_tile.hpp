@@ -303,6 +303,39 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
     set_layout(g->ph[p].rb, bits, n);
     g->ph[p].op0 = 0;
     g->ph[p].nops = 0;
+    // Register order: a phase with the same bits as the one before keeps its
+    // order (no relayout between them).  Otherwise a relayout enters it
+    // anyway, so the order is free: an opening pair of disjoint two-bit
+    // operators goes to positions (2,3) then (0,1), the straight-line
+    // template, and a lone opening pair to (1,2), which the per-operator
+    // dispatch tests first.
+    if (p > 0 && ph_mask[p] == ph_mask[p - 1]) {
+      for (int j = 0; j < kRB; j++) g->ph[p].rb[j] = g->ph[p - 1].rb[j];
+      continue;
+    }
+    int first[2], nf = 0;
+    for (int e = 0; e < ne && nf < 2; e++)
+      if (ph_of[exec[e]] == p) first[nf++] = exec[e];
+    auto pair = [&](int o) { return lk[o] == 2 && popc32(opm[o]) == 2; };
+    if (n < 4 || nf == 0 || !pair(first[0])) continue;
+    int order[kRB], cnt = 0;
+    uint32_t used = 0;
+    auto put = [&](uint32_t m) {
+      for (int b = 0; b < T; b++)
+        if (((m >> b) & 1) && !((used >> b) & 1)) {
+          order[cnt++] = b;
+          used |= 1u << b;
+        }
+    };
+    if (nf == 2 && pair(first[1]) && !(opm[first[0]] & opm[first[1]])) {
+      put(opm[first[1]]);  // positions 0, 1
+      put(opm[first[0]]);  // positions 2, 3
+    } else {
+      put(ph_mask[p] & ~opm[first[0]] & (0u - (ph_mask[p] & ~opm[first[0]])));  // lowest other bit: position 0
+      put(opm[first[0]]);                                                       // positions 1, 2
+    }
+    put(ph_mask[p]);  // the rest, ascending
+    if (cnt == n) set_layout(g->ph[p].rb, order, n);
   }
   g->nph = nph;
   // load/store layouts: the top local bits in registers, unless the first /
