@@ -23,6 +23,8 @@ Beside it, on the same JSON line:
   gate_layer    single-gate layers on an N=30 complex64 state (per-op
                 kernels), achieved GB/s against 8 TB/s; then expect(state, "Z")
                 and expect(state, "X") over every qubit of that state;
+  small_configs the N=12 complex128 brickwork (time per circuit) and the N=8
+                noisy density matrix (complex64 vs complex128, trace, purity);
   serving       Poisson arrivals of mixed circuits through the continuous-
                 batching server: p50/p99 latency and throughput;
   large_state   N=33 complex64 brickwork: 5-qubit blocks on FP32 MFMA vs the
@@ -166,6 +168,45 @@ def observables_on(bt, st, N):
     out["sample"] = dict(shots=int(u.size), ms=round(dt * 1e3, 3), boundary=int(nb),
                          note="bt_sample: chunk sums (one read of the state), prefix, one wavefront per shot; "
                               "wall time including the upload of the uniforms and the download of the indices")
+    return out
+
+
+def small_configs_leg(bt, args):
+    """The two small configurations: the N=12 complex128 brickwork (20 layers,
+    final probabilities) and the N=8 noisy density matrix (16 layers,
+    complex64 against complex128, FP64 trace and purity); time per circuit
+    (state on the device, compiled once)."""
+    import numpy as np
+    from bluetangle_jl_amd import workloads as W
+    from bluetangle_jl_amd.circuit import Circuit
+    out = {}
+    c1 = W.config_c1()
+    cc = Circuit(c1["ops"], c1["N"], False)
+    st = bt.zero_state(c1["N"], "c128")
+    cc.run_on(st.set_basis(0))
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cc.run_on(st.set_basis(0))
+        p = st.probabilities()
+    dt = (time.perf_counter() - t0) / reps
+    out["c1"] = dict(N=c1["N"], dtype="complex128", layers=20, ms_per_circuit=round(dt * 1e3, 3),
+                     prob_sum_minus_1=float(p.sum() - 1.0),
+                     note="bt_run_circuit (register-tile passes) + FP64 probabilities, per circuit")
+    c2 = W.config_c2()
+    cd = Circuit(c2["ops"], c2["N"], c2["noise"])
+    r64, r128 = bt.zero_rho(c2["N"], "c64"), bt.zero_rho(c2["N"], "c128")
+    cd.run_dm(r128)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cd.run_dm(r64.set_basis(0))
+    r64.sync()
+    dt = (time.perf_counter() - t0) / reps
+    tr, pu = r64.trace_purity()
+    err = float(np.abs(r64.to_numpy() - r128.to_numpy()).max())
+    out["c2"] = dict(N=c2["N"], dtype="complex64 (FP64 trace/purity)", layers=16, ms_per_circuit=round(dt * 1e3, 3),
+                     max_entry_err_vs_fp64=err, trace=tr, purity=pu,
+                     note="bt_run_circuit_dm: gates as E rho E^dagger, channels as one superoperator pass each")
     return out
 
 
@@ -321,7 +362,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=24.0)
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--skip", default="", help="comma list of legs to skip: cpu,plain,gate,serve,c5")
+    ap.add_argument("--skip", default="", help="comma list of legs to skip: cpu,plain,gate,small,serve,c5")
     ap.add_argument("--no-c5", dest="c5", action="store_false", help="skip the N=33 large-state leg")
     ap.add_argument("--c5-N", type=int, default=33)
     ap.add_argument("--c5-layers", type=int, default=40)
@@ -387,6 +428,9 @@ def main():
     if "gate" not in skip:
         out["gate_layer"] = gate_layer_leg(bt, args)
         log("gate layer", out["gate_layer"])
+    if "small" not in skip:
+        out["small_configs"] = small_configs_leg(bt, args)
+        log("small configs", out["small_configs"])
     if "serve" not in skip:
         out["serving"] = serving_leg(bt, args)
         log("serving", out["serving"])

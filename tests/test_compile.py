@@ -217,3 +217,18 @@ def test_gpu_to_rho_zne(bt):
         rho = bt.to_rho(c, id=id, prec="c128").to_numpy()
         ref = O.run_dm(N, CR.shot_ops(CR.layered(CR.line_swaps(to_ref(spec))), id=id), nm_ref)
         assert np.abs(rho - ref).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_readme_example(bt):
+    """The README's example runs end to end."""
+    nm = bt.NoiseModel(bt.Noise1("depolarizing", 1e-3), bt.Noise2("depolarizing", 1e-2))
+    ops = [bt.Op("H", 1), bt.Op("CNOT", 1, 2), bt.Op("MZ", 2)]
+    shots = bt.run(ops, 1000, noise=nm, seed=7)
+    assert len(shots) == 1000 and all(len(s) == 1 for s in shots)
+    psi = bt.to_state(ops, noise=nm, seed=7, traj=0)
+    z = bt.expect(psi, "Z")
+    assert len(z) == 2 and abs(abs(z[0]) - 1) < 1e-5  # collapsed Bell pair: both qubits definite
+    assert abs(bt.correlation(psi, "Z,Z", [1, 2]) - 1) < 1e-5
+    m = bt.measure_circuit(ops, 4096, noise=nm, measurement_basis="X")
+    assert m.number_of_experiment == 4096 and abs(m.sample.sum() - 1) < 1e-12
