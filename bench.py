@@ -49,7 +49,7 @@ METRIC = "noisy trajectories/s + achieved HBM GB/s per gate layer; p50/p99 laten
 HBM_PEAK_GBS = 8000.0
 # k_round's measured HBM traffic / algorithmic bytes on this circuit (rocprofv3
 # FETCH_SIZE + WRITE_SIZE, copy-calibrated); re-measured whenever the kernel changes
-TRAFFIC_RATIO = 1.175
+TRAFFIC_RATIO = 1.202
 
 
 def log(*a):
