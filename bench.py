@@ -16,7 +16,9 @@ Beside it, on the same JSON line:
                 inside the library), against 8 TB/s;
   cpu_baseline  the FP64 CPU restatement of the reference (oracle/, fast
                 in-place mode, OpenMP) on a bounded sample of the same
-                circuit;
+                circuit; with it, the same oracle in the reference's own
+                algorithm (sparse full-space operators, one thread) on a
+                sample of that circuit and on the two small configurations;
   plain_loop    the same circuit one trajectory per call (bt_run_circuit: one
                 launch per pass, one circuit per launch);
   per_op_loop   the plain form: apply(state, op; noise) op by op;
@@ -339,9 +341,60 @@ def cpu_leg(args):
     L = int(max(1, min(args.layers, args.cpu_seconds // max(t1, 1e-3))))
     dt = run_layers(L)
     per_traj = dt * args.layers / L
-    return dict(value=round(1.0 / per_traj, 5), unit="trajectories/s", cores=int(O.num_threads()), kind="port",
-                sample="first %d of %d layers of one trajectory of the same N=%d circuit (%.1f s), scaled to a "
-                       "whole trajectory; FP64 in-place strided kernels, OpenMP" % (L, args.layers, args.N, dt))
+    out = dict(value=round(1.0 / per_traj, 5), unit="trajectories/s", cores=int(O.num_threads()), kind="port",
+               sample="first %d of %d layers of one trajectory of the same N=%d circuit (%.1f s), scaled to a "
+                      "whole trajectory; FP64 in-place strided kernels, OpenMP" % (L, args.layers, args.N, dt))
+    if args.cpu_reference:
+        out["reference_algorithm"] = cpu_reference_leg(args, O, nm, brickwork_spec, count_draws_upper, to_ref)
+    return out
+
+
+def cpu_reference_leg(args, O, nm, brickwork_spec, count_draws_upper, to_ref):
+    """The oracle in the reference's own algorithm (every gate, Kraus operator
+    and projector built as a full 2^N x 2^N sparse matrix and applied by a
+    sparse product; hilbert.jl:18-159, struct.jl:31-41), on one thread as the
+    reference runs.  The N=22 trajectory: every 4th op of the first layer (the
+    op mix of a layer), scaled by the circuit's op count.  The two small
+    configurations are timed whole, in this mode and in the in-place mode."""
+    import numpy as np
+    nthreads = int(O.num_threads())
+    O.set_num_threads(1)
+    try:
+        spec1 = brickwork_spec(args.N, 1, 2026, ad_gamma=5e-3, mz_prob=0.1)
+        sub = to_ref(spec1)[::4]
+        total = len(to_ref(brickwork_spec(args.N, args.layers, 2026, ad_gamma=5e-3, mz_prob=0.1)))
+        t0 = time.perf_counter()
+        O.run_traj(args.N, sub, nm, 1, 0, 1, count_draws_upper(spec1, True), mode=1, tol=1e-6)
+        dt = time.perf_counter() - t0
+        per_traj = dt * total / len(sub)
+        res = dict(value=float("%.4g" % (1.0 / per_traj)), unit="trajectories/s", cores=1,
+                   sample="every 4th op of the first layer of the same N=%d circuit (%d ops, %.1f s), scaled by "
+                          "the circuit's %d ops; sparse full-space operators, one thread" % (args.N, len(sub), dt,
+                                                                                              total))
+
+        def per_circuit(fn, min_s=0.5):
+            fn()
+            n, t0 = 0, time.perf_counter()
+            while True:
+                fn()
+                n += 1
+                el = time.perf_counter() - t0
+                if el >= min_s:
+                    return el / n * 1e3
+
+        c1 = to_ref(brickwork_spec(12, 20, 7, gate_noise=False))
+        c2 = to_ref(brickwork_spec(8, 16, 7, gate_noise=True, ad_gamma=5e-3))
+        small = {}
+        for mode, key in ((1, "reference_ms"), (0, "in_place_ms")):
+            small.setdefault("c1", {})[key] = round(per_circuit(
+                lambda: np.abs(O.run_sv(12, c1, None, mode=mode)[0]) ** 2), 3)
+            small.setdefault("c2", {})[key] = round(per_circuit(lambda: O.run_dm(8, c2, nm, mode=mode)), 3)
+        small["note"] = ("time per circuit on one thread: C1 N=12 noiseless brickwork, 20 layers, with |psi|^2; "
+                         "C2 N=8 noisy rho, 16 layers; FP64")
+        res["small_configs"] = small
+        return res
+    finally:
+        O.set_num_threads(nthreads)
 
 
 def main():
@@ -362,6 +415,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=24.0)
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-reference", type=int, default=1,
+                    help="also time the oracle in the reference's sparse full-space-operator algorithm (one thread)")
     ap.add_argument("--skip", default="", help="comma list of legs to skip: cpu,plain,gate,small,serve,c5")
     ap.add_argument("--no-c5", dest="c5", action="store_false", help="skip the N=33 large-state leg")
     ap.add_argument("--c5-N", type=int, default=33)

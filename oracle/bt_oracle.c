@@ -1076,3 +1076,13 @@ int bto_num_threads(void) {
   return 1;
 #endif
 }
+
+/* Thread count of the fast mode's parallel loops (the reference algorithm
+ * mode is timed on one thread, as the reference runs). */
+void bto_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}

@@ -72,6 +72,8 @@ def lib():
         L.bto_weighted_sample.argtypes = [dp, ctypes.c_int, ctypes.c_double]
         L.bto_weighted_sample.restype = ctypes.c_int
         L.bto_num_threads.restype = ctypes.c_int
+        L.bto_set_num_threads.argtypes = [ctypes.c_int]
+        L.bto_set_num_threads.restype = None
         _LIB = L
     return _LIB
 
@@ -496,3 +498,7 @@ def weighted_sample(probs, u):
 
 def num_threads():
     return lib().bto_num_threads()
+
+
+def set_num_threads(n):
+    lib().bto_set_num_threads(int(n))
