@@ -195,7 +195,7 @@ def measure_circuit(circuit_or_ops, number_of_experiment: int, N: Optional[int] 
     if opts.readout_noise is not False and opts.readout_noise is not None:
         # the state-vector path calls apply_noise(state, op, ::QuantumChannel),
         # which has no method in the reference (ops.jl:869 vs hilbert.jl:322)
-        raise L.BTError("readout noise on the state-vector path is not supported by the reference (ops.jl:869)")
+        raise L.BTError(-2, "readout noise on the state-vector path is not supported by the reference (ops.jl:869)")
     if number_of_experiment <= 0:
         raise ValueError("number_of_experiment must be positive")
     n = max(cc.N, N or 0)

@@ -97,7 +97,7 @@ def _expect_product(state: DeviceState, factors: Dict[int, np.ndarray]) -> compl
     for t in terms:
         nterms *= len(t)
     if nterms > _MAX_TERMS:
-        raise L.BTError("expectation: too many non-Pauli factors (%d Pauli terms)" % nterms)
+        raise L.BTError(-1, "expectation: too many non-Pauli factors (%d Pauli terms)" % nterms)
     total = 0j
     for combo in itertools.product(*terms):
         x, z, c = xmask, zmask, scalar

@@ -90,6 +90,20 @@ def test_final_basis_rules():
     assert np.allclose(rot, want, atol=1e-13)
 
 
+def test_readout_noise_on_state_vector_raises(bt):
+    """measure(circuit) with readout noise calls apply_noise(state, op,
+    ::QuantumChannel), which has no method in the reference (ops.jl:869 vs
+    hilbert.jl:322): the engine raises an unsupported-operation error before
+    any device work, and the readout options stay accepted by compile."""
+    from bluetangle_jl_amd import _lib as L
+    ops = [bt.Op("H", 1), bt.Op("CNOT", 1, 2)]
+    with pytest.raises(L.BTError) as e:
+        bt.measure_circuit(ops, 10, readout_noise=0.1)
+    assert e.value.code == -2 and "ops.jl:869" in str(e.value)
+    with pytest.raises(ValueError):
+        bt.measure_circuit(ops, 0)
+
+
 # --- engine (GPU) ------------------------------------------------------------
 @pytest.mark.gpu
 @pytest.mark.parametrize("N,prec", [(10, "c128"), (16, "c64"), (22, "c64")])

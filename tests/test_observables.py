@@ -107,6 +107,20 @@ def test_repeated_qubit_replaces():
     assert np.array_equal(a, b)
 
 
+def test_too_many_dense_factors_is_an_error(bt):
+    """More than three non-Pauli one-qubit factors are expanded into Pauli
+    sums; past 4096 terms the expectation is refused (argument error) before
+    the state is touched."""
+    import importlib
+    obs = importlib.import_module(bt.__name__ + ".observables")
+    from bluetangle_jl_amd import _lib as L
+    A = np.array([[0.3, 0.2 - 0.1j], [0.5j, -0.7]])  # all four Pauli coefficients nonzero
+    factors = {b: A for b in range(10)}  # 3 kept dense, 7 expanded: 4^7 terms
+    with pytest.raises(L.BTError) as e:
+        obs._expect_product(object(), factors)
+    assert e.value.code == -1
+
+
 # --- engine (GPU) ------------------------------------------------------------
 TOL = {"c64": 1e-5, "c128": 1e-12}
 

@@ -200,6 +200,10 @@ def test_sampling_boundary_rules():
     u < prob0 (hilbert.jl:693)."""
     assert O.weighted_sample([0.25, 0.75], 0.25) == 0
     assert O.weighted_sample([0.25, 0.75], 0.2500000001) == 1
+    # Q5: u beyond the FP64 total (the reference returns nothing,
+    # hilbert.jl:814-818): the last branch with positive probability
+    assert O.weighted_sample([0.25, 0.7], 0.99) == 1
+    assert O.weighted_sample([0.5, 0.45, 0.0], 0.999) == 1
     # Born: |+> has prob0 = 1/2 up to rounding; choose a state with exact prob0
     s = np.array([math.sqrt(0.5), 0, math.sqrt(0.5), 0], dtype=complex)  # qubit 1: prob0 = 0.5 exactly? (0.5 + tiny)
     p0 = float(np.real(O.partial_trace(2, s, [1])[0, 0]))
