@@ -1,17 +1,18 @@
 // Dense 5-qubit blocks on FP32 MFMA (complex64 state vectors), and the fused
 // circuit executor built on them.
 //
-// A 32 x 32 complex block acting on 5 bits is, per group of 32 amplitudes,
-// the real product y = A x with A = [[Re M, -Im M], [Im M, Re M]] (64 x 64)
-// and x = (Re, Im) interleaved.  k_fused5 streams the state in tiles of
-// 2^12 amplitudes (the 5 block bits + the 7 lowest other bits): a coalesced
-// load into LDS, then every wavefront multiplies 32 of the tile's 128 groups
-// with v_mfma_f32_32x32x2f32 (the groups as the 32 columns: lanes 0-31
-// supply the real and lanes 32-63 the imaginary part of each component; A is
-// staged once per workgroup in LDS in lane order and read per step), writes
-// its results back over its own groups, and the tile is stored coalesced.
-// 168 VGPRs and 48 KiB of LDS: three workgroups per CU.  Per amplitude: 128
-// real MACs = 16 flop/byte of HBM.
+// A 32 x 32 complex block M acting on 5 bits is, per group of 32 amplitudes
+// x = xr + i xi, y = M x computed with three real 32 x 32 products (Gauss):
+// t1 = Re M xr, t2 = Im M xi, t3 = (Re M + Im M)(xr + xi); then
+// Re y = t1 - t2, Im y = t3 - t1 - t2.  k_fused5 streams the state in tiles
+// of 2^12 amplitudes (the 5 block bits + the 7 lowest other bits): a
+// coalesced load into LDS, then every wavefront multiplies 32 of the tile's
+// 128 groups with v_mfma_f32_32x32x2f32 (the groups as the 32 columns; step s
+// takes components 2s in lanes 0-31 and 2s+1 in lanes 32-63; the three real
+// matrices are staged once per workgroup in LDS in lane order and read per
+// step), writes its results back over its own groups, and the tile is stored
+// coalesced.  44 KiB of LDS: three workgroups per CU.  Per amplitude: 96
+// real MACs (three quarters of the 64 x 64 real form), 12 flop/byte of HBM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +31,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kF5T = 12;  // tile bits
 constexpr int kF5N = 1 << kF5T;
+constexpr int kF5Lane = 3 * 16 * 64;  // floats of one block's MFMA operand (three real matrices)
 
 struct F5Args {
   float2* s;
@@ -46,10 +48,10 @@ __device__ __forceinline__ uint32_t f5_swz(uint32_t l) { return l ^ (((l >> 5) ^
 
 __global__ void __launch_bounds__(256, 3) k_fused5(F5Args a, const float* __restrict__ Alane) {
   __shared__ float2 lds[kF5N];
-  __shared__ float la[64 * 64];  // A in lane order: la[(rb * 32 + i) * 64 + lane]
+  __shared__ float la[kF5Lane];  // la[(t * 16 + s) * 64 + lane], t: Re M, Im M, Re M + Im M
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 #pragma unroll
-  for (int i = 0; i < 16; i++) la[tid + 256 * i] = Alane[tid + 256 * i];
+  for (int i = 0; i < kF5Lane / 256; i++) la[tid + 256 * i] = Alane[tid + 256 * i];
   // this lane's group: g = 32 * wave + lane % 32 -> local base
   const uint32_t grp = 32 * wave + (lane & 31);
   uint32_t gb = 0;
@@ -70,6 +72,8 @@ __global__ void __launch_bounds__(256, 3) k_fused5(F5Args a, const float* __rest
 #pragma unroll
   for (int b = 0; b < 4; b++) dhi[b] = 1ull << a.tile[8 + b];
   const bool hi_half = lane >= 32;
+  // B of step s is component 2s + (lane >= 32): bit 0 of the component is operand 4
+  const uint32_t s1 = s0 ^ (hi_half ? cb[4] : 0u);
   auto tile_base = [&](uint64_t t) {
     uint64_t base = 0;
     for (int r = 0; r < a.nrun; r++) {
@@ -97,34 +101,29 @@ __global__ void __launch_bounds__(256, 3) k_fused5(F5Args a, const float* __rest
     // the next tile's loads fly during this tile's MFMA work (tiles are disjoint)
     if (t + gridDim.x < a.ntiles) fetch(tile_base(t + gridDim.x));
     __syncthreads();
-    floatx16 d0 = {}, d1 = {};
+    floatx16 t1 = {}, t2 = {}, t3 = {};
 #pragma unroll
-    for (int i = 0; i < 32; i++) {
-      // B: component i of this lane's group (real part in lanes 0-31, imaginary in 32-63)
+    for (int st = 0; st < 16; st++) {
       uint32_t sc = 0;
 #pragma unroll
-      for (int k = 0; k < 5; k++)
-        if ((i >> (4 - k)) & 1) sc ^= cb[k];
-      const float2 e = lds[s0 ^ sc];
-      const float bv = hi_half ? e.y : e.x;
-      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(la[i * 64 + lane], bv, d0, 0, 0, 0);
-      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(la[(32 + i) * 64 + lane], bv, d1, 0, 0, 0);
+      for (int k = 0; k < 4; k++)
+        if ((st >> (3 - k)) & 1) sc ^= cb[k];
+      const float2 e = lds[s1 ^ sc];
+      t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(la[st * 64 + lane], e.x, t1, 0, 0, 0);
+      t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(la[(16 + st) * 64 + lane], e.y, t2, 0, 0, 0);
+      t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(la[(32 + st) * 64 + lane], e.x + e.y, t3, 0, 0, 0);
     }
     // each wavefront reads and then overwrites only its own 32 groups: no barrier here
-    // row m of block rb = 8 (j / 4) + 4 (lane / 32) + j % 4: (re, im) of output 16 rb + m / 2
+    // accumulator j holds output component 8 (j / 4) + 4 (lane / 32) + j % 4
 #pragma unroll
-    for (int j = 0; j < 16; j += 2) {
-      const int m = 8 * (j / 4) + (j % 4) + (hi_half ? 4 : 0);
+    for (int j = 0; j < 16; j++) {
+      const int r = 8 * (j / 4) + (j % 4) + (hi_half ? 4 : 0);
+      uint32_t sc = 0;
 #pragma unroll
-      for (int rb = 0; rb < 2; rb++) {
-        const int c = 16 * rb + m / 2;
-        uint32_t sc = 0;
-#pragma unroll
-        for (int i = 0; i < 5; i++)
-          if ((c >> (4 - i)) & 1) sc ^= cb[i];
-        const floatx16& d = rb ? d1 : d0;
-        lds[s0 ^ sc] = make_float2(d[j], d[j + 1]);
-      }
+      for (int i = 0; i < 5; i++)
+        if ((r >> (4 - i)) & 1) sc ^= cb[i];
+      const float re = t1[j] - t2[j];
+      lds[s0 ^ sc] = make_float2(re, t3[j] - t1[j] - t2[j]);
     }
     __syncthreads();
 #pragma unroll
@@ -139,22 +138,21 @@ __global__ void __launch_bounds__(256, 3) k_fused5(F5Args a, const float* __rest
   }
 }
 
-// A (the real 64 x 64 form of a 32 x 32 complex128 column-major block) in
-// MFMA lane order: out[(rb * 32 + i) * 64 + lane] = A[32 rb + lane % 32][2 i + lane / 32].
+// The three real matrices Re M, Im M, Re M + Im M of a 32 x 32 complex128
+// column-major block in MFMA lane order:
+// out[(t * 16 + s) * 64 + lane] = A_t[lane % 32][2 s + lane / 32].
 void fused5_lane_matrix(const double* mat, float* out) {
-  for (int rb = 0; rb < 2; rb++)
-    for (int i = 0; i < 32; i++)
+  for (int t = 0; t < 3; t++)
+    for (int st = 0; st < 16; st++)
       for (int lane = 0; lane < 64; lane++) {
-        const int m = 32 * rb + lane % 32, k = 2 * i + lane / 32;
-        const int r = m / 2, qr = m % 2, c = k / 2, qc = k % 2;
+        const int r = lane % 32, c = 2 * st + lane / 32;
         const double re = mat[2 * (r + c * 32)], im = mat[2 * (r + c * 32) + 1];
-        const double v = (qr == 0) ? (qc == 0 ? re : -im) : (qc == 0 ? im : re);
-        out[(rb * 32 + i) * 64 + lane] = (float)v;
+        out[(t * 16 + st) * 64 + lane] = (float)(t == 0 ? re : (t == 1 ? im : re + im));
       }
 }
 
 // psi <- M psi on 5 bits (bits[0] = MSB); d_alane: M in MFMA lane order on
-// the device (4096 floats).  Enqueued on the state's stream.
+// the device (kF5Lane floats).  Enqueued on the state's stream.
 int launch_fused5(bt_state_s* s, const int* bits, const float* d_alane) {
   const int N = s->n;
   if (s->kind != BT_SV || s->prec != BT_C64 || N < kF5T) return fail(BT_ERR_UNSUPPORTED, "fused5: c64 SV, N >= 12");
@@ -194,7 +192,17 @@ int launch_fused5(bt_state_s* s, const int* bits, const float* d_alane) {
       prev = b;
     }
   a.ntiles = 1ull << (N - kF5T);
-  const unsigned grid = (unsigned)std::min<uint64_t>(a.ntiles, (uint64_t)ctx().cus * 8);
+  // one wave of resident workgroups (each strides over the tiles): a grid
+  // of several partial waves would leave the last one running on part of
+  // the chip
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused5, 256, 0) != hipSuccess || nb < 1) nb = 1;
+    per_cu = nb;
+  }
+  const int cus = ctx().cus > 0 ? ctx().cus : 256;
+  const unsigned grid = (unsigned)std::min<uint64_t>(a.ntiles, (uint64_t)cus * per_cu);
   hipLaunchKernelGGL(k_fused5, dim3(grid), dim3(256), 0, s->stream, a, d_alane);
   BT_HIP(hipGetLastError());
   return BT_OK;
@@ -208,7 +216,7 @@ extern "C" int bt_apply_fused5(bt_state s, const double* mat, const int* bits) {
   int rc = ensure_device();
   if (rc) return rc;
   if (!s || !mat || !bits) return fail(BT_ERR_ARG, "null argument");
-  std::vector<float> al(64 * 64);
+  std::vector<float> al(kF5Lane);
   fused5_lane_matrix(mat, al.data());
   float* d = nullptr;
   BT_HIP(hipMalloc((void**)&d, sizeof(float) * al.size()));
@@ -236,7 +244,7 @@ extern "C" int bt_run_fused(bt_state s, bt_circuit c, int max_k, int norm_every,
   // every 5-bit block's MFMA operand in one device buffer
   float* d_al = nullptr;
   float* h_al = nullptr;
-  const size_t per = 64 * 64;
+  const size_t per = kF5Lane;
   if (mfma && !blocks.empty()) {
     BT_HIP(hipHostMalloc((void**)&h_al, sizeof(float) * per * blocks.size()));
     for (size_t b = 0; b < blocks.size(); b++)
@@ -322,7 +330,7 @@ extern "C" int bt_graph_create_fused(bt_state s, bt_circuit c, int max_k, bt_gra
     return fail(rc, msg);
   }
   g->nblocks = (int64_t)g->blocks.size();
-  const size_t per = 64 * 64;
+  const size_t per = kF5Lane;
   auto cleanup = [&](int r) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
