@@ -195,12 +195,11 @@ int launch_fused5(bt_state_s* s, const int* bits, const float* d_alane) {
   // one wave of resident workgroups (each strides over the tiles): a grid
   // of several partial waves would leave the last one running on part of
   // the chip
-  static int per_cu = 0;
-  if (per_cu == 0) {
+  static const int per_cu = [] {  // thread-safe one-time query (every device runs the same code object)
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused5, 256, 0) != hipSuccess || nb < 1) nb = 1;
-    per_cu = nb;
-  }
+    return nb;
+  }();
   const int cus = ctx().cus > 0 ? ctx().cus : 256;
   const unsigned grid = (unsigned)std::min<uint64_t>(a.ntiles, (uint64_t)cus * per_cu);
   hipLaunchKernelGGL(k_fused5, dim3(grid), dim3(256), 0, s->stream, a, d_alane);
