@@ -349,7 +349,36 @@ def cpu_leg(args):
                       "whole trajectory; FP64 in-place strided kernels, OpenMP" % (L, args.layers, args.N, dt))
     if args.cpu_reference:
         out["reference_algorithm"] = cpu_reference_leg(args, O, nm, brickwork_spec, count_draws_upper, to_ref)
+    out["host"] = host_info()
     return out
+
+
+def host_info():
+    """The host the CPU legs ran on: CPU model, logical CPUs (and those this
+    process may use), RAM, OpenMP threads, the oracle's compiler flags."""
+    model, ram = None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    ram = round(int(line.split()[1]) / 2 ** 20, 1)
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return dict(cpu=model, logical_cpus=os.cpu_count(), usable_cpus=usable, ram_gib=ram,
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                oracle_build="gcc -O2 -std=c11 -fopenmp (oracle/Makefile)",
+                julia_threads="none: no Julia in the image; the reference-algorithm leg runs on one thread, "
+                              "as the reference (which starts no threads)")
 
 
 def cpu_reference_leg(args, O, nm, brickwork_spec, count_draws_upper, to_ref):
