@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "bt_internal.hpp"
+#include "bt_interp.hpp"
 
 namespace bt {
 
@@ -431,10 +432,16 @@ int bt_apply_kraus_traj(bt_state s, int k, int nkraus, const double* kraus, cons
     }
   const double* K = kraus + (size_t)ind * 2 * D * D;
   double nk2 = sandwich_trace(K, rho_a, D);
+  // under the qubit > target ordering quirk the sampled branch can annihilate
+  // the state: the reference's normalize then gives NaN everywhere, and a
+  // norm left over from rounding must not be renormalised into a state
+  double psum = 0;
+  for (int i = 0; i < nkraus; i++) psum += p[i];
+  const double scale = nk2 <= bti::kZeroBranch * psum ? std::nan("") : 1.0 / std::sqrt(nk2);
   Placement pl;
   rc = make_placement(s->n, k, apply_bits, 0, &pl);
   if (rc) return rc;
-  return launch_apply(s, s->n, k, K, pl, 1.0 / std::sqrt(nk2), false);
+  return launch_apply(s, s->n, k, K, pl, scale, false);
 }
 
 int bt_apply_kraus_dm(bt_state s, int k, int nkraus, const double* kraus, const int* bits) {

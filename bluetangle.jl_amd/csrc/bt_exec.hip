@@ -439,7 +439,10 @@ int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats) {
   rc = lower_dm(c->h, &dm, &msg);
   if (rc) return fail(rc, msg);
   int64_t nout = 0;
-  return run_grid(rho, dm, 0, 0, nullptr, 0, nullptr, 0, &nout, stats, 0, tile_bits_for(rho->prec, rho->nbits));
+  // a superoperator spans up to kMaxQK bits of vec(rho): a smaller tile
+  // setting is raised to hold it
+  const int T = std::max(tile_bits_for(rho->prec, rho->nbits), std::min(rho->nbits, bti::kMaxQK));
+  return run_grid(rho, dm, 0, 0, nullptr, 0, nullptr, 0, &nout, stats, 0, T);
 }
 
 int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes, int64_t outcap,

@@ -58,6 +58,9 @@ constexpr int kPMax = 24;     // operators per pass
 constexpr int kMaxT = 15;     // tile bits
 constexpr int kGroupMax = 4;  // decisions resolved from one histogram
 constexpr int kMaxQK = 4;     // widest queued operator (16 x 16: density-matrix superoperators)
+// a sampled Kraus branch whose norm^2 is below this fraction of the state's
+// is taken as annihilating it (exactly zero up to rounding)
+constexpr double kZeroBranch = 1e-10;
 #if defined(__HIP_DEVICE_COMPILE__)
 constexpr int kFuseMax = 2;   // fused products on the device stay 4 x 4 (no scratch arrays)
 #else
@@ -465,20 +468,30 @@ struct Interp {
     }
     plan->write = n > 0 ? 1 : 0;
     plan->aligned = last ? 1 : 0;
-    plan->scale = pend_scale;
-    pend_scale = 1.0;
+    // the pending scale rides on the next pass that writes the state; a
+    // reduction-only pass leaves it pending (its reductions are of the
+    // stored state, and flush_end accounts for the scale still to come)
+    if (plan->write) {
+      plan->scale = pend_scale;
+      pend_scale = 1.0;
+    } else {
+      plan->scale = 1.0;
+    }
     pass_nops = n;
     for (int i = 0; i < n; i++) pass_q[i] = sel[i];
   }
 
-  // The state is aligned with the op stream: N = c^2 |psi_ref|^2.
+  // The state is aligned with the op stream: N is the measured norm^2 of the
+  // stored state phi, and pend_scale^2 N = c^2 |psi_ref|^2 (the pending
+  // scale, 1 after a writing pass, is still to be applied to phi).
   BT_HD void flush_end(double N) {
     norm2 = N;
     if (!fold) return;
+    const double Ne = N * pend_scale * pend_scale;
     if (r_known) {
-      c2 = N / r;
+      c2 = Ne / r;
     } else if (c2_known) {
-      r = N / c2;
+      r = Ne / c2;
       r_known = 1;
     } else {
       err = BT_ERR_STATE;
@@ -630,10 +643,34 @@ struct Interp {
         p[i] = t / tr * r;
       }
       int ind = weighted(p, op.nkraus, d_u, tol_exact);
-      // |K psi|^2 in application order differs from p[ind] only under the
-      // qubit > target ordering quirk; c stays unknown there until the next flush
-      renormalised(op.bits[0] == op.prob_bits[0] ? p[ind] / r : -1.0);
-      push(k, op.bits, 0, P.pool + 2 * (op.mat + (int64_t)ind * D * D), 1.0);
+      const double* Kc = P.pool + 2 * (op.mat + (int64_t)ind * D * D);
+      if (op.bits[0] == op.prob_bits[0]) {
+        renormalised(p[ind] / r);
+        push(k, op.bits, 0, Kc, 1.0);
+        return;
+      }
+      // qubit > target ordering quirk: |K psi|^2 in application order is not
+      // p[ind] (c stays unknown until the next flush), and the sampled branch
+      // can annihilate the state, which the reference's normalize turns into
+      // NaN everywhere; the application-order norm from the same reduced rho
+      // (bit order swapped) tells such a branch from a small one
+      double psum = 0, q = 0;
+      for (int i = 0; i < op.nkraus; i++) psum += p[i];
+      auto sw = [](int a) { return ((a & 1) << 1) | (a >> 1); };
+      for (int r2 = 0; r2 < D; r2++)
+        for (int a = 0; a < D; a++) {
+          const double kr = Kc[2 * (r2 + a * D)], ki = Kc[2 * (r2 + a * D) + 1];
+          if (kr == 0 && ki == 0) continue;
+          for (int b = 0; b < D; b++) {
+            const int sa = k == 2 ? sw(a) : a, sb = k == 2 ? sw(b) : b;
+            double xr = 0, xi = 0;
+            cmulacc(xr, xi, kr, ki, last.rho[2 * (sa + sb * D)], last.rho[2 * (sa + sb * D) + 1]);
+            q += xr * Kc[2 * (r2 + b * D)] + xi * Kc[2 * (r2 + b * D) + 1];
+          }
+        }
+      q = q / tr * r;
+      renormalised(-1.0);
+      push(k, op.bits, 0, Kc, q <= kZeroBranch * psum ? NAN : 1.0);
       return;
     }
     // measurement / if: rho over one bit; rotate by R

@@ -280,7 +280,10 @@ class RoundEngine {
     j->slots.assign(bti::kQMax, QOp());
     bti::Program prog{j->circ->ops.data(), j->circ->ntop, j->circ->pool.data(), j->circ->chan.data(),
                       j->circ->chanw.data(), j->nbits};
-    j->ip->init(prog, j->slots.data(), tile_T(j->nbits), coalesce_bits(prec_));
+    // density-matrix jobs (no norm folding): a superoperator spans up to
+    // kMaxQK bits of vec(rho), so a smaller tile setting is raised to hold it
+    const int T = j->fold ? tile_T(j->nbits) : std::max(tile_T(j->nbits), std::min(j->nbits, bti::kMaxQK));
+    j->ip->init(prog, j->slots.data(), T, coalesce_bits(prec_));
     j->ip->seed = j->seed;
     j->ip->traj = j->traj;
     j->ip->fold = j->fold;

@@ -229,12 +229,9 @@ static int run_tiles(bti::Interp& ip, const bti::QOp* slots, int nbits, std::vec
 }
 
 // stats: passes, reduce passes, boundary draws, draws used, overflow
-extern "C" int emu_run(int nbits, const bt_ir_op* ops, int nops, int ntop, const double* pool, int64_t npool,
-                       uint64_t seed, uint64_t traj, int T, int cmin, int fold, double* state, int* outs,
-                       int64_t outcap, int64_t* nout, int64_t* stats, double tol_exact) {
-  btc::CircuitHost c;
-  int rc = btc::analyze(nbits, ops, nops, ntop, pool, npool, &c, &g_msg);
-  if (rc) return rc;
+static int run_host(const btc::CircuitHost& c, int nbits, uint64_t seed, uint64_t traj, int T, int cmin, int fold,
+                    double* state, int* outs, int64_t outcap, int64_t* nout, int64_t* stats, double tol_exact) {
+  int rc = BT_OK;
   std::vector<bti::QOp> slots(bti::kQMax);
   bti::Interp ip;
   bti::Program prog{c.ops.data(), c.ntop, c.pool.data(), c.chan.data(), c.chanw.data(), nbits};
@@ -328,6 +325,30 @@ extern "C" int emu_run(int nbits, const bt_ir_op* ops, int nops, int ntop, const
   stats[3] = ip.draw;
   stats[4] = ip.overflow;
   return BT_OK;
+}
+
+extern "C" int emu_run(int nbits, const bt_ir_op* ops, int nops, int ntop, const double* pool, int64_t npool,
+                       uint64_t seed, uint64_t traj, int T, int cmin, int fold, double* state, int* outs,
+                       int64_t outcap, int64_t* nout, int64_t* stats, double tol_exact) {
+  btc::CircuitHost c;
+  int rc = btc::analyze(nbits, ops, nops, ntop, pool, npool, &c, &g_msg);
+  if (rc) return rc;
+  return run_host(c, nbits, seed, traj, T, cmin, fold, state, outs, outcap, nout, stats, tol_exact);
+}
+
+// to_rho as the density-matrix executor runs it: the circuit lowered to
+// superoperators on the 2N-bit vector of rho (column-major: bits 0..N-1 the
+// row, N..2N-1 the column), then the same pass machinery.
+extern "C" int emu_run_dm(int nqubits, const bt_ir_op* ops, int nops, int ntop, const double* pool, int64_t npool,
+                          int T, int cmin, double* rho, int64_t* stats) {
+  btc::CircuitHost c, dm;
+  int rc = btc::analyze(nqubits, ops, nops, ntop, pool, npool, &c, &g_msg);
+  if (rc) return rc;
+  rc = btc::lower_dm(c, &dm, &g_msg);
+  if (rc) return rc;
+  int64_t nout = 0;
+  int dummy = 0;
+  return run_host(dm, 2 * nqubits, 0, 0, T, cmin, 0, rho, &dummy, 0, &nout, stats, 1e-9);  // no norm folding: tr rho, not |vec rho|, is 1
 }
 
 // The fusion planner of the fused executor: blocks (k, bits MSB first,
