@@ -606,9 +606,12 @@ int bt_get_tuning(int prec, int* tile_bits, int* coalesce, int* tiles_per_blk) {
   return BT_OK;
 }
 
-// Autotuner of the batched executor: time each candidate (tile bits,
-// coalescing reserve, tiles per workgroup) on `sample` trajectories of the
-// circuit (device time of the round passes) and keep the fastest.
+// Autotuner of the batched executor, in two stages: tile bits and
+// coalescing reserve timed on `sample` trajectories of the circuit (device
+// time of the round passes) at 8 tiles per workgroup, then the tiles per
+// workgroup of the winner on 4 x `sample` trajectories (a run's workgroup
+// count sets what the per-workgroup costs weigh: pass tables staged into LDS,
+// one partial sum each; a small sample hides them).
 int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_tuning* best) {
   if (!c || !best) return fail(BT_ERR_ARG, "null circuit or result");
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
@@ -622,30 +625,42 @@ int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_
     if (T <= nb) Ts.push_back(T);
   if (Ts.empty()) Ts.push_back(std::min(Tmax, nb));
   const int outcap = (int)std::max<size_t>(16, 2 * c->h.ops.size());
-  std::vector<int> outs((size_t)sample * outcap);
-  std::vector<int64_t> nouts(sample);
+  const int64_t sample2 = 4 * sample;
+  std::vector<int> outs((size_t)sample2 * outcap);
+  std::vector<int64_t> nouts(sample2);
   double best_ms = -1;
   bt_tuning b{};
   int rc = BT_OK;
+  auto time_one = [&](int T, int cb, int tpb, int64_t n) -> double {
+    g_tile_override[p] = T;
+    g_coalesce_override[p] = cb;
+    g_tpb_override = tpb;
+    bt_run_stats st;
+    rc = bt_run_batch(c, prec, seed, 0, n, outs.data(), outcap, nouts.data(), nullptr, &st);
+    return rc ? -1.0 : st.kernel_ms / (double)n;
+  };
   for (int T : Ts)
-    for (int cb : {3, 4, 5})
-      for (int tpb : {4, 8, 16}) {
-        if (cb >= T) continue;
-        g_tile_override[p] = T;
-        g_coalesce_override[p] = cb;
-        g_tpb_override = tpb;
-        bt_run_stats st;
-        rc = bt_run_batch(c, prec, seed, 0, sample, outs.data(), outcap, nouts.data(), nullptr, &st);
-        if (rc) break;
-        const double ms = st.kernel_ms / (double)sample;
-        if (best_ms < 0 || ms < best_ms) {
-          best_ms = ms;
-          b.tile_bits = T;
-          b.coalesce_bits = cb;
-          b.tiles_per_block = tpb;
-          b.ms_per_traj = ms;
-        }
+    for (int cb : {3, 4, 5}) {
+      if (cb >= T) continue;
+      const double ms = time_one(T, cb, 8, sample);
+      if (rc) break;
+      if (best_ms < 0 || ms < best_ms) {
+        best_ms = ms;
+        b = bt_tuning{T, cb, 8, ms};
       }
+    }
+  if (rc == BT_OK && best_ms >= 0) {
+    double best2 = -1;
+    for (int tpb : {4, 8, 16, 32}) {
+      const double ms = time_one(b.tile_bits, b.coalesce_bits, tpb, sample2);
+      if (rc) break;
+      if (best2 < 0 || ms < best2) {
+        best2 = ms;
+        b.tiles_per_block = tpb;
+        b.ms_per_traj = ms;
+      }
+    }
+  }
   if (rc || best_ms < 0) {
     g_tile_override[p] = saved[0];
     g_coalesce_override[p] = saved[1];

@@ -249,7 +249,9 @@ typedef struct {
 int bt_set_tuning(int prec, int tile_bits, int coalesce_bits, int tiles_per_block);
 int bt_get_tuning(int prec, int* tile_bits, int* coalesce_bits, int* tiles_per_block);
 /* Autotuner: runs `sample` trajectories of the circuit through bt_run_batch
- * for each candidate setting, keeps (and returns) the fastest. */
+ * for each candidate tile-bits / coalescing setting, then 4 x `sample` for
+ * each tiles-per-workgroup candidate of the winner; keeps (and returns) the
+ * fastest. */
 int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_tuning* best);
 
 /* ---- serving: continuous batching with HBM-budget admission ------------- */
