@@ -50,8 +50,9 @@ sys.path.insert(0, ROOT)
 METRIC = "noisy trajectories/s + achieved HBM GB/s per gate layer; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 # k_round's measured HBM traffic / algorithmic bytes on this circuit (rocprofv3
-# FETCH_SIZE + WRITE_SIZE, copy-calibrated); re-measured whenever the kernel changes
-TRAFFIC_RATIO = 1.202
+# FETCH_SIZE + WRITE_SIZE, copy-calibrated, tiling 13,3,32); re-measured whenever
+# the kernel changes
+TRAFFIC_RATIO = 1.230
 
 
 def log(*a):
