@@ -171,7 +171,7 @@ def test_autotuner_keeps_results(bt):
     c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
     try:
         t = c.autotune("c64", sample=8)
-        assert 6 <= t["tile_bits"] <= 13 and t["coalesce_bits"] in (3, 4, 5) and t["tiles_per_block"] in (4, 8, 16)
+        assert 6 <= t["tile_bits"] <= 13 and t["coalesce_bits"] in (3, 4, 5) and t["tiles_per_block"] in (4, 8, 16, 32)
         assert t["ms_per_traj"] > 0
         outs, states, _ = c.run_batch(12, seed=5, prec="c64", keep_states=True)
         for tr in range(12):
