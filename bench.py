@@ -53,6 +53,11 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE + WRITE_SIZE, copy-calibrated, tiling 13,3,32); re-measured whenever
 # the kernel changes
 TRAFFIC_RATIO = 1.230
+# k_round's FP32 FMA flops per algorithmic byte on this circuit (rocprofv3
+# SQ_INSTS_VALU_FMA_F32 x 64 lanes x 2 over the same launches' algorithmic
+# bytes), and the non-packed FP32 vector peak it is measured against
+FMA_FLOP_PER_BYTE = 11.52
+VALU_FP32_PEAK_TFLOPS = 78.6
 
 
 def log(*a):
@@ -506,7 +511,14 @@ def main():
                                        "ratio measured for this kernel and circuit (profiles/k_round_traffic_pmc.txt)"
                                        % TRAFFIC_RATIO,
                      "launches": int(launches), "avg_launch_ms": round(kms / max(1, launches), 4),
-                     "bytes_per_launch": round(nbytes / max(1, launches), 1)},
+                     "bytes_per_launch": round(nbytes / max(1, launches), 1),
+                     "valu": {"achieved_tflops": round(achieved * FMA_FLOP_PER_BYTE / 1e3, 1),
+                              "peak_tflops": VALU_FP32_PEAK_TFLOPS,
+                              "frac": round(achieved * FMA_FLOP_PER_BYTE / 1e3 / VALU_FP32_PEAK_TFLOPS, 4),
+                              "flop_per_byte": FMA_FLOP_PER_BYTE,
+                              "note": "FP32 FMA work of the operators per algorithmic byte (rocprofv3 "
+                                      "SQ_INSTS_VALU_FMA_F32 x 64 lanes x 2, profiles/k_round_c3_n22_4096traj.txt) "
+                                      "against the non-packed FP32 vector peak (packed FP32 is not used)"}},
     }
     out["precision"] = precision_leg(bt, c, args)
     log("precision", out["precision"])
