@@ -198,9 +198,26 @@ def small_configs_leg(bt, args):
         cc.run_on(st.set_basis(0))
         p = st.probabilities()
     dt = (time.perf_counter() - t0) / reps
+    # the same noiseless circuit as fused <= 3-bit blocks captured once in a
+    # HIP graph on a resident state, replayed per circuit
+    sg = bt.zero_state(c1["N"], "c128")
+    graph = cc.capture(sg, max_k=3)
+    sg.set_basis(0)
+    graph.launch()
+    pg = sg.probabilities()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sg.set_basis(0)
+        graph.launch()
+        pg = sg.probabilities()
+    dtg = (time.perf_counter() - t0) / reps
     out["c1"] = dict(N=c1["N"], dtype="complex128", layers=20, ms_per_circuit=round(dt * 1e3, 3),
                      prob_sum_minus_1=float(p.sum() - 1.0),
-                     note="bt_run_circuit (register-tile passes) + FP64 probabilities, per circuit")
+                     note="bt_run_circuit (register-tile passes) + FP64 probabilities, per circuit",
+                     graph_ms_per_circuit=round(dtg * 1e3, 3), graph_max_prob_diff=float(np.abs(pg - p).max()),
+                     graph_note="fused <= 3-bit blocks captured once in a HIP graph, replayed per circuit "
+                                "(reset + replay + FP64 probabilities)")
+    del graph, sg
     c2 = W.config_c2()
     cd = Circuit(c2["ops"], c2["N"], c2["noise"])
     r64, r128 = bt.zero_rho(c2["N"], "c64"), bt.zero_rho(c2["N"], "c128")
