@@ -609,9 +609,10 @@ int bt_get_tuning(int prec, int* tile_bits, int* coalesce, int* tiles_per_blk) {
 // Autotuner of the batched executor, in two stages: tile bits and
 // coalescing reserve timed on `sample` trajectories of the circuit (device
 // time of the round passes) at 8 tiles per workgroup, then the tiles per
-// workgroup of the winner on 4 x `sample` trajectories (a run's workgroup
-// count sets what the per-workgroup costs weigh: pass tables staged into LDS,
-// one partial sum each; a small sample hides them).
+// workgroup of the winner on 4 x `sample` trajectories, the largest within 1%
+// of the fastest (a run's workgroup count sets what the per-workgroup costs
+// weigh: pass tables staged into LDS, one partial sum each; a sample smaller
+// than the run underweights them).
 int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_tuning* best) {
   if (!c || !best) return fail(BT_ERR_ARG, "null circuit or result");
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
@@ -650,16 +651,21 @@ int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_
       }
     }
   if (rc == BT_OK && best_ms >= 0) {
+    const int tpbs[4] = {4, 8, 16, 32};
+    double t2[4];
     double best2 = -1;
-    for (int tpb : {4, 8, 16, 32}) {
-      const double ms = time_one(b.tile_bits, b.coalesce_bits, tpb, sample2);
-      if (rc) break;
-      if (best2 < 0 || ms < best2) {
-        best2 = ms;
-        b.tiles_per_block = tpb;
-        b.ms_per_traj = ms;
-      }
+    for (int i = 0; i < 4 && rc == BT_OK; i++) {
+      t2[i] = time_one(b.tile_bits, b.coalesce_bits, tpbs[i], sample2);
+      if (rc == BT_OK && (best2 < 0 || t2[i] < best2)) best2 = t2[i];
     }
+    // the largest tiles per workgroup within 1% of the best: the sample
+    // still has fewer workgroups than a full run, whose per-workgroup costs
+    // it underweights
+    for (int i = 0; i < 4 && rc == BT_OK; i++)
+      if (t2[i] <= best2 * 1.01) {
+        b.tiles_per_block = tpbs[i];
+        b.ms_per_traj = t2[i];
+      }
   }
   if (rc || best_ms < 0) {
     g_tile_override[p] = saved[0];

@@ -355,10 +355,21 @@ class RoundEngine {
     pending_ = false;
     if (act.empty()) return BT_OK;
     // tiles per workgroup: enough to amortise the workgroup's set-up, few
-    // enough that the largest item still spreads over the chip
+    // enough that the largest item still spreads over the chip and that the
+    // launch has workgroups for several waves of the chip's slots — a
+    // workgroup walks its tiles in turn, so in a launch with little total
+    // work (a serving round) many tiles per workgroup only lengthen it
     uint32_t max_tiles = 0;
-    for (Job* j : act) max_tiles = std::max<uint32_t>(max_tiles, (uint32_t)(1ull << (j->nbits - j->plan.ntile)));
-    const int tpb = (int)std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)tiles_per_block(), max_tiles / 32));
+    uint64_t total_tiles = 0;
+    for (Job* j : act) {
+      const uint64_t nt = 1ull << (j->nbits - j->plan.ntile);
+      max_tiles = std::max<uint32_t>(max_tiles, (uint32_t)nt);
+      total_tiles += nt;
+    }
+    const int cus = ctx().cus > 0 ? ctx().cus : 256;
+    const uint64_t by_total = total_tiles / (uint64_t)(8 * cus);
+    const int tpb = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>({(uint64_t)tiles_per_block(), (uint64_t)(max_tiles / 32), by_total}));
     size_t blob = 0;
     uint32_t blocks = 0;
     std::vector<RoundItem> items(act.size());
