@@ -205,6 +205,16 @@ static int g_tpb_override = 0;
 
 int tiles_per_block() { return g_tpb_override ? g_tpb_override : 8; }
 
+// Tiles per workgroup a rounds launch uses: the setting, capped so that the
+// largest item still spreads over at least 32 workgroups and the launch has
+// about eight workgroups per CU (a workgroup walks its tiles in turn, so a
+// launch with little total work would otherwise run long on few workgroups).
+int effective_tpb(int setting, uint64_t max_tiles, uint64_t total_tiles) {
+  const int cus = ctx().cus > 0 ? ctx().cus : 256;
+  const uint64_t by_total = total_tiles / (uint64_t)(8 * cus);
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)setting, max_tiles / 32, by_total}));
+}
+
 int default_tile_bits(int prec) {
   return prec == BT_C64 ? TileCfg<float>::LOGNT + TileCfg<float>::RB : TileCfg<double>::LOGNT + TileCfg<double>::RB;
 }
@@ -612,7 +622,12 @@ int bt_get_tuning(int prec, int* tile_bits, int* coalesce, int* tiles_per_blk) {
 // workgroup of the winner on 4 x `sample` trajectories, the largest within 1%
 // of the fastest (a run's workgroup count sets what the per-workgroup costs
 // weigh: pass tables staged into LDS, one partial sum each; a sample smaller
-// than the run underweights them).
+// than the run underweights them).  Stage 2 times each distinct effective
+// value once (effective_tpb: candidates the caps turn into the same launch
+// are one candidate) and installs and reports the effective value; a
+// candidate that the sample's total work would cap below what a full run
+// uses is not timed unless no candidate is left.  The first failing run
+// ends the search with its error and the previous tuning restored.
 int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_tuning* best) {
   if (!c || !best) return fail(BT_ERR_ARG, "null circuit or result");
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
@@ -632,7 +647,9 @@ int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_
   double best_ms = -1;
   bt_tuning b{};
   int rc = BT_OK;
+  // rc keeps the first error: no candidate runs after it
   auto time_one = [&](int T, int cb, int tpb, int64_t n) -> double {
+    if (rc) return -1.0;
     g_tile_override[p] = T;
     g_coalesce_override[p] = cb;
     g_tpb_override = tpb;
@@ -642,6 +659,7 @@ int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_
   };
   for (int T : Ts)
     for (int cb : {3, 4, 5}) {
+      if (rc) break;
       if (cb >= T) continue;
       const double ms = time_one(T, cb, 8, sample);
       if (rc) break;
@@ -651,19 +669,30 @@ int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_
       }
     }
   if (rc == BT_OK && best_ms >= 0) {
-    const int tpbs[4] = {4, 8, 16, 32};
+    const uint64_t max_tiles = 1ull << (nb - std::min(nb, b.tile_bits));
+    const uint64_t total = (uint64_t)sample2 * max_tiles;
+    int cand[4], nc = 0;
+    for (int pass = 0; pass < 2 && nc == 0; pass++)
+      for (int want : {4, 8, 16, 32}) {
+        const int eff = effective_tpb(want, max_tiles, total);
+        const int uncapped = effective_tpb(want, max_tiles, ~0ull >> 1);
+        if (pass == 0 && eff != uncapped) continue;  // the sample's size caps it below a full run's
+        bool dup = false;
+        for (int i = 0; i < nc; i++) dup = dup || cand[i] == eff;
+        if (!dup) cand[nc++] = eff;
+      }
     double t2[4];
     double best2 = -1;
-    for (int i = 0; i < 4 && rc == BT_OK; i++) {
-      t2[i] = time_one(b.tile_bits, b.coalesce_bits, tpbs[i], sample2);
+    for (int i = 0; i < nc && rc == BT_OK; i++) {
+      t2[i] = time_one(b.tile_bits, b.coalesce_bits, cand[i], sample2);
       if (rc == BT_OK && (best2 < 0 || t2[i] < best2)) best2 = t2[i];
     }
     // the largest tiles per workgroup within 1% of the best: the sample
     // still has fewer workgroups than a full run, whose per-workgroup costs
     // it underweights
-    for (int i = 0; i < 4 && rc == BT_OK; i++)
+    for (int i = 0; i < nc && rc == BT_OK; i++)
       if (t2[i] <= best2 * 1.01) {
-        b.tiles_per_block = tpbs[i];
+        b.tiles_per_block = cand[i];
         b.ms_per_traj = t2[i];
       }
   }

@@ -100,7 +100,7 @@ struct Plan {
 struct PassResult {
   double norm2;
   double hist[16];
-  double rho[32];        // up to 4x4 complex, column-major
+  double rho[128];       // up to 8x8 complex, column-major
 };
 
 struct Program {
@@ -842,11 +842,7 @@ struct Interp {
             pc++;
             break;
           }
-          if (op.k > 2) {
-            err = BT_ERR_UNSUPPORTED;
-            return 0;
-          }
-          // exact: reduced rho over the probability bits
+          // exact: reduced rho over the probability bits (up to 3)
           d_pc = pc;
           d_u = u;
           draw++;

@@ -35,6 +35,7 @@ namespace bt {
 int tile_bits_for(int prec, int nbits);
 int coalesce_bits(int prec);
 int tiles_per_block();
+int effective_tpb(int setting, uint64_t max_tiles, uint64_t total_tiles);
 
 // ---------------------------------------------------------------------------
 // device side
@@ -99,12 +100,13 @@ __global__ void __launch_bounds__(TileCfg<R>::NT, TileCfg<R>::WAVES) k_round(con
   if (threadIdx.x < kAcc) part[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = sums[threadIdx.x];
 }
 
-// out[item, j]: fixed-order sum of the item's partials (grid: items x kAcc)
+// out[item, j]: fixed-order sum of the item's partials (grid: items x nacc,
+// the accumulators the round's reductions use)
 __global__ void __launch_bounds__(256) k_round_finalize(const RoundItem* items, const double* part, uint64_t stride,
                                                         double* out) {
   const RoundItem it = items[blockIdx.x];
   const double t = sum_blocks(part + (uint64_t)blockIdx.y * stride + it.block0, it.nblocks);
-  if (threadIdx.x == 0) out[(uint64_t)blockIdx.x * kAcc + blockIdx.y] = t;
+  if (threadIdx.x == 0) out[(uint64_t)blockIdx.x * gridDim.y + blockIdx.y] = t;
 }
 
 template <typename R>
@@ -366,10 +368,7 @@ class RoundEngine {
       max_tiles = std::max<uint32_t>(max_tiles, (uint32_t)nt);
       total_tiles += nt;
     }
-    const int cus = ctx().cus > 0 ? ctx().cus : 256;
-    const uint64_t by_total = total_tiles / (uint64_t)(8 * cus);
-    const int tpb = (int)std::max<uint64_t>(
-        1, std::min<uint64_t>({(uint64_t)tiles_per_block(), (uint64_t)(max_tiles / 32), by_total}));
+    const int tpb = effective_tpb(tiles_per_block(), max_tiles, total_tiles);
     size_t blob = 0;
     uint32_t blocks = 0;
     std::vector<RoundItem> items(act.size());
@@ -413,10 +412,13 @@ class RoundEngine {
     }
     BT_HIP(hipGetLastError());
     BT_HIP(hipEventRecord(ev_b_, stream_));
-    hipLaunchKernelGGL(k_round_finalize, dim3((unsigned)items.size(), kAcc), dim3(256), 0, stream_, d_items_, d_part_,
-                       (uint64_t)blocks, d_out_);
+    nacc_ = kAccSmall;
+    for (Job* j : act)
+      if (j->plan.red == bti::RED_RHO && j->plan.red_k == 3) nacc_ = kAcc;
+    hipLaunchKernelGGL(k_round_finalize, dim3((unsigned)items.size(), (unsigned)nacc_), dim3(256), 0, stream_,
+                       d_items_, d_part_, (uint64_t)blocks, d_out_);
     BT_HIP(hipGetLastError());
-    BT_HIP(hipMemcpyAsync(h_out_, d_out_, sizeof(double) * kAcc * items.size(), hipMemcpyDeviceToHost, stream_));
+    BT_HIP(hipMemcpyAsync(h_out_, d_out_, sizeof(double) * nacc_ * items.size(), hipMemcpyDeviceToHost, stream_));
     BT_HIP(hipEventRecord(ev_done_, stream_));
     pending_ = true;
     launched_blocks_ += blocks;
@@ -433,7 +435,7 @@ class RoundEngine {
     // 4. hand the results back
     for (size_t i = 0; i < act_.size(); i++) {
       Job* j = act_[i];
-      to_result(j->plan, j->red_swap, h_out_ + kAcc * i, &j->res);
+      to_result(j->plan, j->red_swap, h_out_ + (size_t)nacc_ * i, &j->res);
       if (j->plan.write) j->fresh = 0;
     }
     return BT_OK;
@@ -499,6 +501,7 @@ class RoundEngine {
   double* d_out_ = nullptr;
   double* h_out_ = nullptr;
   int64_t rounds_ = 0;
+  int nacc_ = kAccSmall;
   uint64_t launched_blocks_ = 0;
   hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
   double pass_ms_ = 0;

@@ -12,7 +12,7 @@
 // addresses are bit deposits of (tile index, thread, register) - no tables -
 // and the layouts used to load and store keep the low local bits on the
 // threads, so each wavefront touches contiguous 64 B+ runs.  Reductions (norm,
-// histogram over up to 4 bits, reduced rho over up to 2 bits) run in FP64
+// histogram over up to 4 bits, reduced rho over up to 3 bits) run in FP64
 // and are added per tile in a fixed order.
 #pragma once
 
@@ -30,7 +30,11 @@ using bti::QOp;
 namespace bt {
 
 constexpr int kMatEntries = bti::Interp::kPassMatEntries;
-constexpr int kAcc = 33;   // norm2, hist[16], packed rho[16]
+// accumulators of a pass: [0] norm^2, then either the histogram (16 bins) or
+// the packed reduced rho (diagonal, then Re/Im of rho[a, b] for a < b: 16
+// doubles over 2 bits, 64 over 3)
+constexpr int kAcc = 65;
+constexpr int kAccSmall = 17;  // enough unless a pass reduces rho over 3 bits
 constexpr int kRB = 5;     // register bits (most): 32 amplitudes per thread
 constexpr int kMaxPh = 28;
 
@@ -114,6 +118,8 @@ struct PassGeom {
   int32_t red_bits[4];       // RED_HIST: global bits; RED_RHO: register positions (last phase), ascending
   int8_t io_in[kRB], io_out[kRB];
   int8_t pad[6];
+  uint32_t red_loff[8];      // RED_RHO over 3 bits: local offset of each index (red_bits[0] = MSB)
+  int32_t red_ls[4];         // RED_RHO over 3 bits: the three local bits, ascending
   LSlots lay[kMaxPh + 2];    // slot tables: phases, then kLayIn, kLayOut
   double scale;
   RPhase ph[kMaxPh];
@@ -280,7 +286,11 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
     done |= best;
   }
   uint32_t redm = 0;
-  if (plan.red == bti::RED_RHO) {
+  if (plan.red == bti::RED_RHO && plan.red_k == 3) {
+    // read from the tile image in LDS after the last phase: no register
+    // positions needed, but the pass needs a phase to end
+    if (nph == 0) ph_mask[nph++] = 0;
+  } else if (plan.red == bti::RED_RHO) {
     for (int i = 0; i < plan.red_k; i++) redm |= 1u << rank[plan.red_bits[i]];
     if (nph == 0 || popc32(cur | redm) > nrb) {
       if (nph >= kMaxPh) return BT_ERR_UNSUPPORTED;
@@ -498,6 +508,24 @@ __host__ __device__ __attribute__((noinline)) int build_geom(const Plan& plan, c
   // 5. reductions
   if (plan.red == bti::RED_HIST) {
     for (int i = 0; i < plan.red_k; i++) g->red_bits[i] = plan.red_bits[i];
+  } else if (plan.red == bti::RED_RHO && plan.red_k == 3) {
+    for (int a = 0; a < 8; a++) {
+      uint32_t off = 0;
+      for (int i = 0; i < 3; i++)
+        if ((a >> (2 - i)) & 1) off |= 1u << rank[plan.red_bits[i]];
+      g->red_loff[a] = off;
+    }
+    int ls[3];
+    for (int i = 0; i < 3; i++) ls[i] = rank[plan.red_bits[i]];
+    for (int i = 0; i < 3; i++)
+      for (int j = i + 1; j < 3; j++)
+        if (ls[j] < ls[i]) {
+          const int t = ls[i];
+          ls[i] = ls[j];
+          ls[j] = t;
+        }
+    for (int i = 0; i < 3; i++) g->red_ls[i] = ls[i];
+    g->red_ls[3] = 0;
   } else if (plan.red == bti::RED_RHO) {
     const RPhase& ph = g->ph[nph - 1];
     int p[2] = {0, 0};
@@ -696,6 +724,47 @@ __host__ __device__ __forceinline__ void apply_lds(C<R>* lds, int T, const ROp& 
   }
 }
 
+// Reduced rho over three tile bits, read from the tile image in LDS
+// (swizzled): thread tid takes packed entry e = tid / 4 (the 8 diagonal
+// entries, then the 28 pairs a < b in order (0,1), (0,2), ..., (6,7)) and the
+// index groups tid % 4, tid % 4 + 4, ...; returns e (-1: no entry) and the
+// thread's partial sum of Re (xr) and Im (xi) of rho[a, b].
+template <typename R>
+__host__ __device__ __forceinline__ int rho3_partial(const C<R>* lds, const PassGeom& g, uint32_t tid, double* xr,
+                                                     double* xi) {
+  const int e = (int)(tid >> 2), s = (int)(tid & 3);
+  *xr = 0.0;
+  *xi = 0.0;
+  if (e >= 36) return -1;
+  int a = e, b = e;
+  if (e >= 8) {
+    int p = e - 8;
+    a = 0;
+    while (p >= 7 - a) {
+      p -= 7 - a;
+      a++;
+    }
+    b = a + 1 + p;
+  }
+  const uint32_t oa = g.red_loff[a], ob = g.red_loff[b];
+  const uint32_t ng = 1u << (g.T - 3);
+  double sr = 0.0, si = 0.0;
+  for (uint32_t grp = (uint32_t)s; grp < ng; grp += 4) {
+    uint32_t lb = grp;
+#pragma unroll
+    for (int i = 0; i < 3; i++) lb = ((lb >> g.red_ls[i]) << (g.red_ls[i] + 1)) | (lb & ((1u << g.red_ls[i]) - 1u));
+    const C<R> va = lds[swz<R>(lb + oa)], vb = lds[swz<R>(lb + ob)];
+    sr += (double)va.x * (double)vb.x + (double)va.y * (double)vb.y;
+    si += (double)va.y * (double)vb.x - (double)va.x * (double)vb.y;
+  }
+  *xr = sr;
+  *xi = si;
+  return e;
+}
+
+// packed index of entry e's real part (its imaginary part follows for e >= 8)
+__host__ __device__ __forceinline__ int rho3_slot(int e) { return e < 8 ? e : 8 + 2 * (e - 8); }
+
 #if defined(__HIPCC__)
 // A double moved across lanes by a DPP pattern (both 32-bit halves).
 template <int CTRL>
@@ -724,6 +793,24 @@ __device__ __forceinline__ double wave_sum(double x) {
   x += dpp_f64<0x141>(x);  // row_half_mirror
   x += dpp_f64<0x140>(x);  // row_mirror
   return (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
+}
+
+// The quad's four partials of a 3-bit reduced-rho entry, summed in a fixed
+// order ((s0 + s1) + (s2 + s3)), added by the quad's first lane to its
+// wavefront's packed-rho slots (wrho: LDS; each entry has one writer).
+template <typename R>
+__device__ __forceinline__ void rho3_accumulate_t(const C<R>* lds, const PassGeom& g, double* wrho) {
+  double xr, xi;
+  const int e = rho3_partial<R>(lds, g, threadIdx.x, &xr, &xi);
+  xr += dpp_f64<0xb1>(xr);
+  xr += dpp_f64<0x4e>(xr);
+  xi += dpp_f64<0xb1>(xi);
+  xi += dpp_f64<0x4e>(xi);
+  if (e >= 0 && (threadIdx.x & 3) == 0) {
+    const int sl = rho3_slot(e);
+    wrho[sl] += xr;
+    if (e >= 8) wrho[sl + 1] += xi;
+  }
 }
 
 // Add M per-thread values into this wavefront's accumulator slots
@@ -1143,11 +1230,18 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
         }
       }
       if (p == g.nph - 1 && g.red == bti::RED_RHO) {
-        double racc[16];
+        if (uflag(g.red_k == 3)) {
+          regs_to_lds<R>(v, lds, g.lay[ci], io.tid, io.active, io.nvalid_mask);
+          __syncthreads();
+          rho3_accumulate_t<R>(lds, g, wred + 1);
+          __syncthreads();
+        } else {
+          double racc[16];
 #pragma unroll
-        for (int j = 0; j < 16; j++) racc[j] = 0.0;
-        if (io.active) rho_mask<R>(rho_reg_mask(g), v, io.nvalid_mask, racc);
-        wave_accumulate<16>(racc, wred + 17);
+          for (int j = 0; j < 16; j++) racc[j] = 0.0;
+          if (io.active) rho_mask<R>(rho_reg_mask(g), v, io.nvalid_mask, racc);
+          wave_accumulate<16>(racc, wred + 1);
+        }
       }
     }
     double acc[17];
@@ -1197,9 +1291,9 @@ __host__ __device__ inline void unpack_rho(int k, const double* packed, double* 
 
 __host__ __device__ inline void to_result(const Plan& plan, int red_swap, const double* sum, PassResult* res) {
   res->norm2 = sum[0];
-  for (int j = 0; j < 16; j++) res->hist[j] = sum[1 + j];
-  for (int j = 0; j < 32; j++) res->rho[j] = 0.0;
-  if (plan.red == bti::RED_RHO) unpack_rho(plan.red_k, sum + 17, res->rho, red_swap);
+  for (int j = 0; j < 16; j++) res->hist[j] = plan.red == bti::RED_HIST ? sum[1 + j] : 0.0;
+  for (int j = 0; j < 128; j++) res->rho[j] = 0.0;
+  if (plan.red == bti::RED_RHO) unpack_rho(plan.red_k, sum + 1, res->rho, red_swap);
 }
 
 }  // namespace bt

@@ -173,12 +173,26 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
           for (int tid = 0; tid < NT; tid++) lds_to_regs<R>(v[tid].data(), lds.data(), curS, io[tid].ltid);
         }
       }
-      if (p == g.nph - 1 && g.red == bti::RED_RHO) {
+      if (p == g.nph - 1 && g.red == bti::RED_RHO && g.red_k == 3) {
+        // as the device: the tile image in LDS, per-thread partials, the
+        // quad's four summed (s0 + s1) + (s2 + s3)
+        for (int tid = 0; tid < NT; tid++)
+          regs_to_lds<R>(v[tid].data(), lds.data(), curS, tid, io[tid].active, io[tid].nvalid_mask);
+        for (int q = 0; q < NT / 4; q++) {
+          double xr[4], xi[4];
+          int e = -1;
+          for (int s = 0; s < 4; s++) e = rho3_partial<R>(lds.data(), g, (uint32_t)(4 * q + s), &xr[s], &xi[s]);
+          if (e < 0) continue;
+          const int sl = rho3_slot(e);
+          sums[1 + sl] += (xr[0] + xr[1]) + (xr[2] + xr[3]);
+          if (e >= 8) sums[2 + sl] += (xi[0] + xi[1]) + (xi[2] + xi[3]);
+        }
+      } else if (p == g.nph - 1 && g.red == bti::RED_RHO) {
         for (int tid = 0; tid < NT; tid++) {
           double racc[16];
           for (int j = 0; j < 16; j++) racc[j] = 0.0;
           if (io[tid].active) rho_mask<R>(rho_reg_mask(g), v[tid].data(), io[tid].nvalid_mask, racc);
-          for (int j = 0; j < 16; j++) sums[17 + j] += racc[j];
+          for (int j = 0; j < 16; j++) sums[1 + j] += racc[j];
         }
       }
     }
@@ -294,7 +308,7 @@ static int run_host(const btc::CircuitHost& c, int nbits, uint64_t seed, uint64_
       }
       for (uint64_t base = 0; base < len; base++) {
         if (base & m) continue;
-        cd v[4];
+        cd v[8];
         for (int a = 0; a < D; a++) {
           uint64_t o = 0;
           for (int j = 0; j < k; j++)

@@ -6,7 +6,7 @@ engine's tables), so a mistake in one table cannot hide in both.
 
   ("gate", name, qubit, target, control, noisy)
   ("chan", model, p, qubit, target)            # OpQC(model, p, qubit[, target])
-  ("kraus", [K...], qubit, target)             # OpQC("custom", kraus, qubit[, target])
+  ("kraus", [K...], qubit, target)             # OpQC("custom", kraus, qubit[, target]); 1-3 qubits
   ("meas", name, qubit)                        # Op("MZ"/"MX"/"MY"/"MR", qubit)
   ("reset", qubit)                             # Op("RES", qubit)
   ("ifop", name, qubit, spec0, spec1)          # ifOp(name, qubit, if0, if1)
@@ -91,6 +91,40 @@ def count_draws_upper(spec, noisy_model):
     return n
 
 
+def random_unitary(D, rng):
+    z = rng.normal(size=(D, D)) + 1j * rng.normal(size=(D, D))
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def random_kraus(q, rng, style=None):
+    """A random CPTP Kraus list on q qubits (2^q x 2^q), one of three kinds:
+    'mix' (sqrt(w_i) U_i: state-independent probabilities), 'general'
+    (A_i S^-1/2 with S = sum A_i'A_i: probabilities need the reduced rho) or
+    'damping' (amplitude damping on each qubit, tensored: decided from
+    eigenvalue bounds unless the draw is close to 1)."""
+    D = 2 ** q
+    style = style or ["mix", "general", "damping"][rng.integers(3)]
+    if style == "mix":
+        n = int(rng.integers(2, 4))
+        w = rng.dirichlet(np.ones(n))
+        return [math.sqrt(wi) * random_unitary(D, rng) for wi in w]
+    if style == "general":
+        n = int(rng.integers(1, 4))
+        A = [rng.normal(size=(D, D)) + 1j * rng.normal(size=(D, D)) for _ in range(n)]
+        S = sum(a.conj().T @ a for a in A)
+        ev, V = np.linalg.eigh(S)
+        Sm = (V * ev ** -0.5) @ V.conj().T
+        return [a @ Sm for a in A]
+    g = float(rng.uniform(0.02, 0.3))
+    k1 = [np.array([[1, 0], [0, math.sqrt(1 - g)]], dtype=complex),
+          np.array([[0, math.sqrt(g)], [0, 0]], dtype=complex)]
+    ks = k1
+    for _ in range(q - 1):
+        ks = [np.kron(a, b) for a in ks for b in k1]
+    return ks
+
+
 def random_spec(N, depth, rng, kinds=("gate1", "gate2"), nonadjacent=True, controls=True):
     """Random gate lists in the spirit of random_ops (gates.jl:154-214),
     drawn from our own seeded numpy generator."""
@@ -136,6 +170,17 @@ def random_spec(N, depth, rng, kinds=("gate1", "gate2"), nonadjacent=True, contr
                 # pick a zero-norm branch (NaN in the reference too); keep q < t here.
                 q, t = t, q
             spec.append(("chan", model, float(rng.uniform(0.05, 0.4)), q, t))
+        elif k == "kraus":
+            # OpQC(name, kraus, qubit[, target]) with a random CPTP list on 1, 2
+            # or 3 qubits (struct.jl:182-234; 3 qubits act on qubit..qubit+2)
+            q = int(rng.integers(1, min(N, 3) + 1))
+            if q == 1:
+                spec.append(("kraus", random_kraus(1, rng), int(rng.integers(1, N + 1)), -1))
+            elif q == 2:
+                a, b = [int(x) for x in rng.choice(np.arange(1, N + 1), 2, replace=False)]
+                spec.append(("kraus", random_kraus(2, rng), a, b))
+            else:
+                spec.append(("kraus", random_kraus(3, rng), int(rng.integers(1, N - 1)), -1))
         elif k == "meas":
             spec.append(("meas", ["MZ", "MX", "MY", "MR"][rng.integers(4)], int(rng.integers(1, N + 1))))
         elif k == "reset":

@@ -158,3 +158,68 @@ def test_opf_matrix_and_op_list(bt, prec):
     assert np.abs(s1.to_numpy() - r).max() < TOL[prec]
     with pytest.raises(Exception):
         bt.apply(bt.zero_rho(3, prec), bt.OpF("U", np.eye(8)))
+
+
+def _f1(theta, phi):
+    """RZ(phi) RY(theta), written out (a function-typed gate's f(pars...))."""
+    c, s = math.cos(theta / 2), math.sin(theta / 2)
+    e0, e1 = np.exp(-0.5j * phi), np.exp(0.5j * phi)
+    return np.array([[c * e0, -s * e0], [s * e1, c * e1]])
+
+
+def _f2(theta, phi):
+    """exp(-i theta Z(x)Z / 2) (I (x) RZ(phi)): diagonal, written out."""
+    z = np.array([1, -1, -1, 1])
+    rz = np.array([np.exp(-0.5j * phi), np.exp(0.5j * phi)] * 2)
+    return np.diag(np.exp(-0.5j * theta * z) * rz)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_function_gates_and_opf(bt, prec):
+    """Function-typed gates evaluated with `pars` and placed like any gate,
+    noise after them (hilbert.jl:499-500, struct.jl:518-527), and OpF with a
+    function of the state or an op list, applied without noise
+    (struct.jl:703-744, hilbert.jl:487-488), against the oracle's run of the
+    same matrices and ops."""
+    pars = (0.83, -1.27)
+    N = 5
+
+    def fn(state):  # an OpF function: a small circuit on the device state
+        return bt.apply(state, [bt.Op("H", 1), bt.Op("CNOT", 1, 2), bt.Op("RY(0.4)", 4)])
+
+    ops = [bt.Op("RX(0.3)", 3), bt.Op("f1", _f1, 2), bt.OpF("fn", fn), bt.Op("MZ", 1), bt.Op("f2", _f2, 1, 3),
+           bt.OpF("lst", [bt.Op("RX(0.7)", 5), bt.Op("CZ", 4, 5)]), bt.Op("f1", _f1, 3, control=1),
+           bt.Op("f2", _f2, 4, 2), bt.Op("MX", 2), bt.Op("MZ", 3)]
+    ref = [O.ref_gate("RX(0.3)", 3), O.ref_gate("f1", 2, mat=_f1(*pars)),
+           O.ref_gate("H", 1, noisy=False), O.ref_gate("CNOT", 1, 2, noisy=False), O.ref_gate("RY(0.4)", 4, noisy=False),
+           O.ref_gate("MZ", 1), O.ref_gate("f2", 1, 3, mat=_f2(*pars)),
+           O.ref_gate("RX(0.7)", 5, noisy=False), O.ref_gate("CZ", 4, 5, noisy=False),
+           O.ref_gate("f1", 3, control=1, mat=_f1(*pars)), O.ref_gate("f2", 4, 2, mat=_f2(*pars)),
+           O.ref_gate("MX", 2), O.ref_gate("MZ", 3)]
+    nm = ("depolarizing", 0.05, "amplitude_damping", 0.1)
+    checked = 0
+    for seed in range(6):
+        bt.seed(seed)
+        st, outs = bt.apply(bt.zero_state(N, prec), ops, noise=bt.NoiseModel(*nm), track_measurements=True,
+                            pars=pars)
+        u = O.uniforms(seed, 0, 64)
+        r, routs, _, bnd, _ = O.run_sv(N, ref, O.ref_noise(*nm), u, tol=1e-6)
+        if bnd:
+            continue
+        assert outs == routs, seed
+        assert np.abs(st.to_numpy() - r).max() < TOL[prec], seed
+        checked += 1
+    assert checked >= 4
+    # the circuit executors evaluate function-typed gates with the circuit's pars
+    from bluetangle_jl_amd.circuit import Circuit
+    gops = [o for o in ops if not isinstance(o, bt.OpF)]
+    gref = [r for r in ref if r.noisy or r.kind != O.KIND_GATE]
+    c = Circuit(gops, N, bt.NoiseModel(*nm), pars=pars)
+    outs, states, _ = c.run_batch(4, seed=11, prec=prec, keep_states=True)
+    for t in range(4):
+        r, routs, _, bnd, _ = O.run_sv(N, gref, O.ref_noise(*nm), O.uniforms(11, t, 64), tol=1e-6)
+        if bnd:
+            continue
+        assert outs[t] == routs and np.abs(states[t] - r).max() < TOL[prec], t
+    with pytest.raises(ValueError):
+        bt.apply(bt.zero_state(N, prec), bt.Op("f1", _f1, 2))  # a function gate needs pars

@@ -171,11 +171,83 @@ def test_autotuner_keeps_results(bt):
     c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
     try:
         t = c.autotune("c64", sample=8)
-        assert 6 <= t["tile_bits"] <= 13 and t["coalesce_bits"] in (3, 4, 5) and t["tiles_per_block"] in (4, 8, 16, 32)
+        # N=12 is one tile per trajectory: every candidate runs one tile per workgroup
+        assert 6 <= t["tile_bits"] <= 13 and t["coalesce_bits"] in (3, 4, 5) and t["tiles_per_block"] == 1
         assert t["ms_per_traj"] > 0
         outs, states, _ = c.run_batch(12, seed=5, prec="c64", keep_states=True)
         for tr in range(12):
             r, routs, bnd = oracle_traj(N, spec, nm_args, 5, tr)
+            if bnd:
+                continue
+            assert outs[tr] == routs
+            assert np.abs(states[tr] - r).max() < 1e-5
+    finally:
+        _lib.check(_lib.lib().bt_set_tuning(0, 0, 0, 0))
+
+
+def test_autotuner_times_what_it_installs(bt):
+    """At N=23 with 64 trajectories per stage-2 sample the engine's caps do
+    not bind: the chosen tiles per workgroup is one of the timed candidates,
+    installed as is."""
+    from bluetangle_jl_amd import _lib
+    from bluetangle_jl_amd.circuit import Circuit
+    import ctypes
+    N = 23
+    spec = brickwork_spec(N, 2, 4, ad_gamma=5e-3, mz_prob=0.1)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel("depolarizing", 1e-3, "depolarizing", 1e-2))
+    try:
+        t = c.autotune("c64", sample=16)
+        assert t["tiles_per_block"] in (4, 8, 16, 32), t
+        got = [ctypes.c_int() for _ in range(3)]
+        _lib.check(_lib.lib().bt_get_tuning(0, *[ctypes.byref(g) for g in got]))
+        assert [g.value for g in got] == [t["tile_bits"], t["coalesce_bits"], t["tiles_per_block"]]
+    finally:
+        _lib.check(_lib.lib().bt_set_tuning(0, 0, 0, 0))
+
+
+def test_autotuner_error_restores_tuning(bt):
+    """A circuit that no pass can hold (a 4-bit gate with a control inside the
+    tile: five operand bits) makes the first candidate fail: the autotuner
+    returns that error and leaves the tuning as it was."""
+    from bluetangle_jl_amd import _lib as L
+    from bluetangle_jl_amd.circuit import IR
+    import ctypes
+    ir = IR(6)
+    ir.ops = [ir.op(L.BT_OP_GATE, [0, 1, 2, 3], ctrl=1 << 4, mat=ir.add_matrix(np.eye(16)))]
+    ir.ntop = 1
+    arr, pool = ir.arrays()
+    h = ctypes.c_void_p()
+    L.check(L.lib().bt_circuit_create(6, arr, 1, 1, L.dptr(pool), pool.size // 2, ctypes.byref(h)))
+    try:
+        L.check(L.lib().bt_set_tuning(0, 12, 4, 16))
+        t = L.Tuning()
+        rc = L.lib().bt_autotune_batch(h, 0, 4, 1, ctypes.byref(t))
+        assert rc != 0
+        got = [ctypes.c_int() for _ in range(3)]
+        L.check(L.lib().bt_get_tuning(0, *[ctypes.byref(g) for g in got]))
+        assert [g.value for g in got] == [12, 4, 16]
+    finally:
+        L.check(L.lib().bt_set_tuning(0, 0, 0, 0))
+        L.lib().bt_circuit_destroy(h)
+
+
+@pytest.mark.parametrize("tpb", [3, 5])
+def test_rounds_with_partial_workgroups(bt, tpb):
+    """Tiles per workgroup that do not divide a trajectory's tiles (256 tiles
+    of 2^6 amplitudes at N=14; 3 or 5 per workgroup, below both caps of the
+    engine) leave each trajectory's last workgroup partly filled: outcomes
+    and amplitudes still match the oracle."""
+    from bluetangle_jl_amd import _lib
+    from bluetangle_jl_amd.circuit import Circuit
+    N = 14
+    spec = brickwork_spec(N, 6, 8, ad_gamma=5e-3, mz_prob=0.2)
+    nm_args = ("depolarizing", 1e-3, "depolarizing", 1e-2)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
+    try:
+        _lib.check(_lib.lib().bt_set_tuning(0, 6, 3, tpb))
+        outs, states, _ = c.run_batch(64, seed=9, prec="c64", keep_states=True)
+        for tr in range(0, 64, 8):
+            r, routs, bnd = oracle_traj(N, spec, nm_args, 9, tr)
             if bnd:
                 continue
             assert outs[tr] == routs

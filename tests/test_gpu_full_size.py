@@ -10,7 +10,9 @@ depend on a CPU run of the same size (the oracle cannot hold or finish them):
   same outcomes as inside the full batch; single trajectories through the
   one-circuit executor, in complex64 and complex128, give the batch's
   outcomes (draws within the boundary tolerance excepted) and states of
-  fidelity 1 - 1e-6 with each other.
+  fidelity 1 - 1e-6 with each other.  One trajectory of it (N=22 is within
+  the oracle's reach: about a minute on 16 host threads) is also compared
+  with the FP64 oracle directly: identical outcomes, amplitudes within 1e-5.
 * N=33 complex64 brickwork (64 GiB): 4 layers as fused 5-qubit MFMA blocks,
   then the adjoint circuit through the register-tile executor, return the
   state to |0..0> (every <Z_q> = 1); the fused and the register-tile results
@@ -21,6 +23,8 @@ import math
 
 import numpy as np
 import pytest
+
+import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -123,3 +127,30 @@ def test_c5_full_size_fused_and_adjoint(bt):
     finally:
         del a
         gc.collect()
+
+
+def test_c3_trajectory_vs_oracle(bt):
+    """One trajectory of the batched workload at its full size (N=22, 32
+    layers, depolarizing 1e-3/1e-2, amplitude damping 5e-3, MZ p=0.1; complex64
+    amplitudes) from bt_run_batch against the FP64 oracle's run of the same
+    circuit on the same uniforms (ops.jl:659-692): identical outcomes, max
+    amplitude error within 1e-5."""
+    from bluetangle_jl_amd import workloads as W
+    from bluetangle_jl_amd.circuit import Circuit
+    from refcirc import brickwork_spec, count_draws_upper, to_ref
+    cfg = W.config_c3()
+    N = cfg["N"]
+    c = Circuit(cfg["ops"], N, cfg["noise"])
+    seed, ntraj = 5, 4
+    outs, states, st = c.run_batch(ntraj, seed=seed, prec="c64", keep_states=True)
+    spec = brickwork_spec(N, 32, 2026, ad_gamma=5e-3, mz_prob=0.1)
+    assert len(spec) == len(cfg["ops"])
+    nm = O.ref_noise("depolarizing", 1e-3, "depolarizing", 1e-2)
+    ro, rs, bnd = O.run_traj(N, to_ref(spec), nm, seed, 0, 1, count_draws_upper(spec, True), mode=0, tol=1e-6,
+                             keep_states=True)
+    if bnd:
+        pytest.skip("a draw within 1e-6 of a cumulative boundary")
+    assert outs[0] == ro[0]
+    err = float(np.abs(states[0] - rs[0]).max())
+    assert err < 1e-5, err
+    print("C3 trajectory 0 vs oracle: %d outcomes identical, max amplitude error %.3g" % (len(ro[0]), err))

@@ -113,3 +113,82 @@ def test_budget_admission_and_rejection(bt):
         assert srv2.fetch(rid, probs=False)[0] == outs_b[t]
     srv2.close()
     srv.close()
+
+
+def test_upper_sizes_mix_equals_alone(bt):
+    """The serving configuration at the top of its size ranges: a state-vector
+    request at N=28 (2 GiB complex64) and a density-matrix request at N=13
+    (4^13 entries, 512 MiB complex64), with brickwork, depolarizing
+    1e-3/1e-2, amplitude damping 5e-3 and MZ p=0.1 (state vector), beside
+    small requests: each result is bit-identical alone and in the mix, the
+    state vector's probabilities sum to 1 and the density matrix's diagonal
+    (its trace) to 1."""
+    from bluetangle_jl_amd.circuit import Circuit, Server
+    from bluetangle_jl_amd import workloads as W
+    nm = W.depolarizing_noise()
+    big_sv = Circuit(W.brickwork_ops(28, 12, 41, ad_gamma=5e-3, mz_prob=0.1), 28, nm)
+    big_dm = Circuit(W.brickwork_ops(13, 6, 42, ad_gamma=5e-3), 13, nm)
+    small = Circuit(W.brickwork_ops(10, 20, 43, ad_gamma=5e-3, mz_prob=0.1), 10, nm)
+    small_dm = Circuit(W.brickwork_ops(7, 10, 44, ad_gamma=5e-3), 7, nm)
+    reqs = [(big_sv, "sv", "c64", 3, 0), (small, "sv", "c64", 3, 1), (big_dm, "dm", "c64", 0, 0),
+            (small, "sv", "c128", 3, 2), (small_dm, "dm", "c64", 0, 0)]
+    srv = Server()
+    ids = [srv.submit(c, k, p, s, t) for c, k, p, s, t in reqs]
+    res = _drain(srv, len(ids))
+    mixed = {}
+    for rid in ids:
+        assert res[rid]["status"] == 0, res[rid]
+        mixed[rid] = srv.fetch(rid)
+    srv.close()
+    o, p = mixed[ids[0]]
+    assert p.size == 2 ** 28 and abs(p.sum() - 1.0) < 1e-5 and len(o) > 0
+    assert abs(mixed[ids[2]][1].sum() - 1.0) < 1e-5
+    for rid, (c, k, p, s, t) in zip(ids, reqs):
+        s1 = Server()
+        r1 = s1.submit(c, k, p, s, t)
+        _drain(s1, 1)
+        outs, probs = s1.fetch(r1)
+        s1.close()
+        assert outs == mixed[rid][0], (k, p, c.N)
+        assert np.array_equal(probs, mixed[rid][1]), (k, p, c.N)
+        del probs
+    mixed.clear()
+
+
+def test_arrivals_during_execution(bt):
+    """Requests submitted while earlier ones are on the device (the server is
+    mid-way through them) are admitted as rounds go, and every result is the
+    same as the request alone."""
+    import time
+    from bluetangle_jl_amd.circuit import Circuit, Server
+    from bluetangle_jl_amd import workloads as W
+    nm = W.depolarizing_noise()
+    long_c = Circuit(W.brickwork_ops(22, 60, 51, ad_gamma=5e-3, mz_prob=0.1), 22, nm)
+    short = [Circuit(W.brickwork_ops(n, 10, 52 + n, ad_gamma=5e-3, mz_prob=0.1), n, nm) for n in (8, 12, 16)]
+    srv = Server()
+    first = [srv.submit(long_c, "sv", "c64", 7, t) for t in range(4)]
+    t0 = time.time()
+    while srv.stats()["active"] == 0 and time.time() - t0 < 30:
+        time.sleep(0.001)
+    assert srv.stats()["active"] > 0
+    early = srv.poll()
+    later = []
+    for i in range(12):
+        c = short[i % 3]
+        later.append((srv.submit(c, "sv", "c64", 7, 100 + i), c, 100 + i))
+    busy = srv.stats()
+    done = {r["id"]: r for r in early}
+    while len(done) < len(first) + len(later):
+        for r in srv.poll(wait=True):
+            done[r["id"]] = r
+    # the long requests were still running when the short ones arrived
+    assert len(early) < len(first) and busy["active"] + busy["queued"] > 0
+    results = {rid: srv.fetch(rid) for rid in done}
+    srv.close()
+    for rid, c, traj in [(r, long_c, t) for t, r in enumerate(first)] + later:
+        s1 = Server()
+        r1 = s1.submit(c, "sv", "c64", 7, traj)
+        _drain(s1, 1)
+        outs, probs = s1.fetch(r1)
+        s1.close()
+        assert outs == results[rid][0] and np.array_equal(probs, results[rid][1]), (c.N, traj)

@@ -93,6 +93,64 @@ def test_random_noisy_circuits(emu, bt, seed, T, mode):
     assert np.allclose(s, r, rtol=0, atol=1e-11 if mode < 2 else 1e-5, equal_nan=True)
 
 
+KRAUS_KINDS = ("gate1", "gate2", "kraus", "kraus", "meas", "chan1")
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("seed", range(16))
+@pytest.mark.parametrize("T", [3, 5])
+def test_random_custom_kraus_circuits(emu, bt, seed, T, mode):
+    """OpQC with random custom Kraus lists on 1, 2 and 3 qubits (unitary
+    mixtures, general CPTP lists whose branch probabilities need the reduced
+    rho over 1-3 bits, tensored amplitude damping) against the oracle, through
+    the pass plans and the register-tile passes (the 3-bit reduced rho is read
+    from the tile image in LDS)."""
+    rng = np.random.default_rng(7000 + seed)
+    N = 3 + seed % 5
+    spec = random_spec(N, 40, rng, kinds=KRAUS_KINDS)
+    nm_args = ("depolarizing", 0.05, "amplitude_damping", 0.1)
+    tol = 1e-9 if mode < 2 else 1e-6
+    r, routs, used, bnd = oracle_run(spec, N, nm_args, seed, tol=tol)
+    if bnd:
+        pytest.skip("boundary draw")
+    emu.emu_set_exec(mode)
+    try:
+        s, outs, stats = emu_run(emu, bt, spec, N, bt.NoiseModel(*nm_args), seed, T=T, cmin=1, tol=tol)
+    finally:
+        emu.emu_set_exec(0)
+    assert outs == routs
+    assert stats[3] == used
+    assert np.array_equal(np.isnan(s), np.isnan(r))
+    assert np.allclose(s, r, rtol=0, atol=1e-11 if mode < 2 else 1e-5, equal_nan=True)
+
+
+def test_three_qubit_channel_needs_exact_rho(emu, bt):
+    """A general 3-qubit CPTP list (probabilities from the 3-bit reduced rho,
+    struct.jl:44-55) placed where the state is entangled across the three
+    qubits: the decision is exact, every branch index agrees with the oracle
+    over many trajectories."""
+    from refcirc import random_kraus
+    rng = np.random.default_rng(99)
+    N = 5
+    ks = random_kraus(3, rng, "general")
+    spec = [("gate", "H", 1, -1, -2, True), ("gate", "CNOT", 1, 2, -2, True), ("gate", "RY(0.7)", 3, -1, -2, True),
+            ("gate", "CNOT", 2, 3, -2, True), ("gate", "RX(1.3)", 4, -1, -2, True),
+            ("kraus", ks, 2, -1), ("meas", "MZ", 3), ("kraus", ks, 1, -1), ("meas", "MX", 2)]
+    for mode in (0, 1, 2):
+        for traj in range(12):
+            tol = 1e-9 if mode < 2 else 1e-6
+            r, routs, used, bnd = oracle_run(spec, N, None, 5, traj, tol=tol)
+            if bnd:
+                continue
+            emu.emu_set_exec(mode)
+            try:
+                s, outs, stats = emu_run(emu, bt, spec, N, False, 5, traj, T=4, cmin=1, tol=tol)
+            finally:
+                emu.emu_set_exec(0)
+            assert outs == routs and stats[3] == used
+            assert np.allclose(s, r, rtol=0, atol=1e-11 if mode < 2 else 1e-5)
+
+
 # A measurement branch whose probability is decided lazily (the state's
 # scale c unknown), then a gate that is not exactly unitary (the reference's
 # T, entries rounded to 10 digits): settling c needs a pass that only

@@ -188,3 +188,40 @@ def test_gpu_server_qasm(bt, qz):
     o2, _ = Circuit(cc.ops(), cc.N, nm).run_on(st, seed=3, traj=5)
     assert outs == o2
     assert np.abs(probs - st.probabilities()).max() < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_gpu_qasm_vs_oracle(bt, qz, prec):
+    """run(qasm) on the device against the oracle's run of the parsed program
+    (qasm.jl:285-421 import, then vcat(layers...) as run applies it,
+    ops.jl:659-692): the same outcome lists, and each shot's final state
+    within 1e-12 (complex128) / 1e-5 (complex64), with a noise model."""
+    from bluetangle_jl_amd.circuit import Circuit, compiled
+    text = RUNNABLE + "measure q[0] -> c[0]; reset q[1]; measure q[2] -> c[2]; h q[3]; measure q[3] -> c[3];"
+    nm = bt.NoiseModel(bt.Noise1("amplitude_damping", 0.05), bt.Noise2("depolarizing", 0.05))
+    rnm = O.ref_noise("amplitude_damping", 0.05, "depolarizing", 0.05)
+    shots, seed = 16, 21
+    outs = bt.run(text, shots, noise=nm, seed=seed, prec=prec)
+    cc = compiled(text, nm)
+    ops = cc.ops()
+    ref = []
+    for o in ops:
+        if isinstance(o, bt.OpQC):
+            ref.append(O.ref_reset(o.qubit))
+        elif o.type == "🔬":
+            ref.append(O.ref_gate(o.name, o.qubit))
+        else:
+            # the SWAPs that compile inserts for routing are noiseless (noisy=False)
+            ref.append(O.ref_gate(o.name, o.qubit, o.target_qubit, o.control, noisy=o.noisy, mat=np.asarray(o.mat)))
+    _, states, _ = Circuit(ops, cc.N, nm).run_batch(shots, seed=seed, prec=prec, keep_states=True)
+    tol = 1e-12 if prec == "c128" else 1e-5
+    checked = 0
+    for t in range(shots):
+        r, routs, _, bnd, _ = O.run_sv(cc.N, ref, rnm, O.uniforms(seed, t, 256), tol=1e-6)
+        if bnd:
+            continue
+        assert outs[t] == routs, t
+        assert np.abs(states[t] - r).max() < tol, t
+        checked += 1
+    assert checked >= shots - 2
