@@ -25,7 +25,7 @@ SYMBOLS = [
     "bt_last_error", "bt_version", "bt_init", "bt_uniform",
     "bt_state_create", "bt_state_destroy", "bt_state_info", "bt_state_set_basis", "bt_state_copy",
     "bt_state_upload", "bt_state_download", "bt_state_device_ptr", "bt_state_sync", "bt_state_bytes",
-    "bt_apply_full", "bt_apply_fused5", "bt_run_fused", "bt_inner", "bt_expect", "bt_sample", "bt_run_batch_sample", "bt_graph_create_fused", "bt_graph_launch",
+    "bt_apply_full", "bt_apply_fused5", "bt_run_fused", "bt_inner", "bt_expect", "bt_sample", "bt_sample_noalloc", "bt_run_batch_sample", "bt_graph_create_fused", "bt_graph_launch",
     "bt_graph_passes", "bt_graph_destroy",
     "bt_apply_matrix", "bt_reduced_rho", "bt_apply_kraus_traj", "bt_apply_kraus_dm", "bt_measure", "bt_reset",
     "bt_probabilities", "bt_norm2", "bt_trace_purity", "bt_gather",
@@ -108,6 +108,7 @@ def lib():
             "bt_inner": (ctypes.c_int, [vp, vp, dp, dp]),
             "bt_expect": (ctypes.c_int, [vp, ctypes.c_int, dp, ip, ctypes.c_uint64, ctypes.c_uint64, dp, dp]),
             "bt_sample": (ctypes.c_int, [vp, ctypes.c_int64, dp, u64p, i64p]),
+            "bt_sample_noalloc": (ctypes.c_int, [vp, ctypes.c_int64, dp, u64p, i64p]),
             "bt_run_batch_sample": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
                                                    ip, ctypes.c_int64, i64p, u64p, i64p, ctypes.POINTER(RunStats)]),
             "bt_graph_create_fused": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.POINTER(vp)]),

@@ -44,6 +44,8 @@ int ensure_device() {
   hipDeviceProp_t prop;
   BT_HIP(hipGetDeviceProperties(&prop, c.device));
   c.cus = prop.multiProcessorCount;
+  const int rc = noalloc_setup(c.device);  // the no-allocation sampler's scratch address on this device
+  if (rc) return rc;
   c.ready = true;
   return BT_OK;
 }

@@ -104,6 +104,9 @@ struct DeviceCtx {
 };
 DeviceCtx& ctx();
 int ensure_device();
+// address of the no-allocation entry point's static scratch on `device`
+// (hipGetSymbolAddress, once per device, with the device current)
+int noalloc_setup(int device);
 
 }  // namespace bt
 

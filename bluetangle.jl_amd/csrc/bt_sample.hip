@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "bt_internal.hpp"
@@ -186,9 +187,91 @@ int launch_sample(const void* state, int prec, int nbits, hipStream_t st, void* 
   return BT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// the no-allocation variant's scratch: one static device array per GPU
+// ---------------------------------------------------------------------------
+// Holds the chunk sums and prefix, the shots' uniforms, indices and boundary
+// flags, and the boundary count: 32 MiB covers a 2^30-amplitude state (8 MiB
+// of sums and prefix) with about 1.4 million shots.  In host code the array's
+// name is not a device address: kernels and copies take the address that
+// hipGetSymbolAddress returned for the current device (noalloc_setup).
+constexpr size_t kNoAllocScratch = 32ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (the library's device set-up).
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(BT_ERR_ARG, "noalloc_setup: device index out of range");
+  if (g_scratch_ptr[device]) return BT_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(BT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return BT_OK;
+}
+
+// Number of flagged shots, written (not accumulated) to *cnt: one workgroup.
+__global__ void __launch_bounds__(256) k_count_flags(const uint8_t* flag, int64_t shots, int64_t* cnt) {
+  __shared__ int64_t red[256];
+  int64_t t = 0;
+  for (int64_t i = threadIdx.x; i < shots; i += 256) t += flag[i];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *cnt = red[0];
+}
+
 }  // namespace bt
 
 using namespace bt;
+
+extern "C" int bt_sample_noalloc(bt_state s, int64_t shots, const double* u, uint64_t* out, int64_t* nboundary) {
+  if (!s) return fail(BT_ERR_ARG, "null state");
+  if (s->kind != BT_SV) return fail(BT_ERR_UNSUPPORTED, "bt_sample_noalloc: state vectors only");
+  if (shots < 0 || (shots > 0 && (!u || !out))) return fail(BT_ERR_ARG, "bt_sample_noalloc: bad arguments");
+  if (shots == 0) {
+    if (nboundary) *nboundary = 0;
+    return BT_OK;
+  }
+  for (int64_t i = 0; i < shots; i++)
+    if (!(u[i] >= 0.0 && u[i] < 1.0)) return fail(BT_ERR_ARG, "bt_sample_noalloc: uniforms must lie in [0, 1)");
+  const size_t sc = (sample_scratch_bytes(s->nbits) + 255) / 256 * 256;
+  const size_t ub = sizeof(double) * (size_t)shots, ob = sizeof(uint64_t) * (size_t)shots, fb = (size_t)shots;
+  const size_t need = sc + ub + ob + (fb + 7) / 8 * 8 + sizeof(int64_t);
+  if (need > kNoAllocScratch)
+    return fail(BT_ERR_ARG, "bt_sample_noalloc: state and shots need " + std::to_string(need) +
+                                " bytes of scratch, more than its " + std::to_string(kNoAllocScratch) +
+                                " (use bt_sample)");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices || !g_scratch_ptr[dev])
+    return fail(BT_ERR_STATE, "bt_sample_noalloc: the device's scratch is not set up (bt_init)");
+  char* base = (char*)g_scratch_ptr[dev];
+  double* d_u = (double*)(base + sc);
+  uint64_t* d_out = (uint64_t*)(base + sc + ub);
+  uint8_t* d_flag = (uint8_t*)(base + sc + ub + ob);
+  int64_t* d_cnt = (int64_t*)(base + sc + ub + ob + (fb + 7) / 8 * 8);
+  int64_t nb = 0;
+  int rc = BT_OK;
+  if (hipMemcpyAsync(d_u, u, ub, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    rc = fail(BT_ERR_HIP, "bt_sample_noalloc: upload");
+  if (rc == BT_OK) rc = launch_sample(s->d, s->prec, s->nbits, s->stream, base, shots, d_u, 0.0, d_out, d_flag);
+  if (rc == BT_OK) {
+    hipLaunchKernelGGL(k_count_flags, dim3(1), dim3(256), 0, s->stream, (const uint8_t*)d_flag, shots, d_cnt);
+    if (hipGetLastError() != hipSuccess) rc = fail(BT_ERR_HIP, "bt_sample_noalloc: count launch");
+  }
+  if (rc == BT_OK && (hipMemcpyAsync(out, d_out, ob, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+                      hipMemcpyAsync(&nb, d_cnt, sizeof nb, hipMemcpyDeviceToHost, s->stream) != hipSuccess))
+    rc = fail(BT_ERR_HIP, "bt_sample_noalloc: download");
+  if (hipStreamSynchronize(s->stream) != hipSuccess && rc == BT_OK) rc = fail(BT_ERR_HIP, "bt_sample_noalloc: sync");
+  if (rc) return rc;
+  if (nboundary) *nboundary = nb;
+  return BT_OK;
+}
 
 extern "C" int bt_sample(bt_state s, int64_t shots, const double* u, uint64_t* out, int64_t* nboundary) {
   if (!s) return fail(BT_ERR_ARG, "null state");

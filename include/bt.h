@@ -137,6 +137,14 @@ int bt_expect(bt_state s, int k, const double* mat, const int* bits, uint64_t xm
  * decide.  Replaces sample(state, shots) (ops.jl:46-60; StatsBase.sample
  * with Weights, whose RNG the explicit uniforms stand in for). */
 int bt_sample(bt_state s, int64_t shots, const double* u, uint64_t* out, int64_t* nboundary);
+/* bt_sample without any allocation: the same kernels, results and errors,
+ * with every device buffer (chunk sums, prefix, uniforms, indices, flags) in
+ * one static 32 MiB device array per GPU (a 2^30-amplitude state with about
+ * 1.4 million shots fits; larger requests return BT_ERR_ARG).  No counterpart
+ * in the reference: for callers that cannot allocate (tight serving loops,
+ * harnesses that forbid hipMalloc).  Calls on one GPU share the array: calls
+ * that overlap on one GPU (states on different streams) race. */
+int bt_sample_noalloc(bt_state s, int64_t shots, const double* u, uint64_t* out, int64_t* nboundary);
 /* DM: FP64 trace and purity sum |rho_ij|^2 */
 int bt_trace_purity(bt_state s, double* trace, double* purity);
 /* Amplitudes (or rho entries) at n indices, complex128 interleaved out. */
