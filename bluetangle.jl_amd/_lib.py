@@ -30,7 +30,7 @@ SYMBOLS = [
     "bt_apply_matrix", "bt_reduced_rho", "bt_apply_kraus_traj", "bt_apply_kraus_dm", "bt_measure", "bt_reset",
     "bt_probabilities", "bt_norm2", "bt_trace_purity", "bt_gather",
     "bt_circuit_create", "bt_circuit_destroy", "bt_circuit_max_draws", "bt_run_circuit", "bt_run_circuit_dm",
-    "bt_run_batch", "bt_run_batch_persistent", "bt_set_tile_bits", "bt_set_tuning", "bt_get_tuning",
+    "bt_run_batch", "bt_run_batch_persistent", "bt_release_cache", "bt_set_tile_bits", "bt_set_tuning", "bt_get_tuning",
     "bt_autotune_batch",
     "bt_server_create", "bt_server_submit", "bt_server_poll", "bt_server_fetch", "bt_server_stats",
     "bt_server_destroy",
@@ -153,6 +153,7 @@ def lib():
             "bt_server_fetch": (ctypes.c_int, [vp, ctypes.c_int64, ip, ctypes.c_int64, i64p, dp, ctypes.c_uint64]),
             "bt_server_stats": (ctypes.c_int, [vp, ip, ip, u64p]),
             "bt_server_destroy": (ctypes.c_int, [vp]),
+            "bt_release_cache": (ctypes.c_int, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name, None)

@@ -183,9 +183,13 @@ int bt_state_create(int nqubits, int kind, int prec, bt_state* out) {
   s->nbits = nbits;
   s->len = 1ull << nbits;
   if (hipMalloc(&s->d, s->len * s->elem()) != hipSuccess) {
-    delete s;
     (void)hipGetLastError();
-    return fail(BT_ERR_NOMEM, "hipMalloc of the state failed (out of HBM)");
+    // the batch executor's cached slab may hold the memory: release it, try again
+    if (!release_cached_memory() || hipMalloc(&s->d, s->len * s->elem()) != hipSuccess) {
+      delete s;
+      (void)hipGetLastError();
+      return fail(BT_ERR_NOMEM, "hipMalloc of the state failed (out of HBM)");
+    }
   }
   if (hipMalloc(&s->d_part, sizeof(double) * kRedBlocks * kRedMax) != hipSuccess ||
       hipMalloc(&s->d_out, sizeof(double) * kRedMax) != hipSuccess ||

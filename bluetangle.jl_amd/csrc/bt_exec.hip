@@ -474,6 +474,7 @@ int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0
   int64_t nwg = std::min<int64_t>(ntraj, (int64_t)cus * per_cu);
   // keep the per-workgroup states within a quarter of free HBM
   size_t freeb = 0, totalb = 0;
+  (void)release_cached_memory();  // the batch executor's cached slab would otherwise hold the HBM
   BT_HIP(hipMemGetInfo(&freeb, &totalb));
   const size_t per_wg = len * elem + sizeof(QOp) * bti::kQMax + sizeof(bti::Interp);
   nwg = std::max<int64_t>(1, std::min<int64_t>(nwg, (int64_t)(freeb / 4 / per_wg)));

@@ -107,6 +107,9 @@ int ensure_device();
 // address of the no-allocation entry point's static scratch on `device`
 // (hipGetSymbolAddress, once per device, with the device current)
 int noalloc_setup(int device);
+// free the batch executor's cached state slab (unless a batch is running);
+// returns the bytes released
+size_t release_cached_memory();
 
 }  // namespace bt
 

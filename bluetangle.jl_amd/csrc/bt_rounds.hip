@@ -453,6 +453,11 @@ class RoundEngine {
 
   int64_t rounds() const { return rounds_; }
   double pass_ms() const { return pass_ms_; }
+  void reset_counters() {
+    rounds_ = 0;
+    pass_ms_ = 0;
+    launched_blocks_ = 0;
+  }
 
  private:
   int reserve(size_t blob, size_t nitems, uint32_t blocks) {
@@ -528,7 +533,75 @@ static int export_state(Job* j, hipStream_t st, double* host, double* d_tmp) {
 
 }  // namespace bt
 
+namespace bt {
+
+// What a batch needs besides its inputs and outputs, kept between
+// bt_run_batch calls: the two cohort engines of each precision (their worker
+// thread pools, stream, events and pinned staging) and the slab that holds
+// the resident states.  Building them costs tens of milliseconds of host time
+// (30 worker threads, a slab of up to 3/4 of HBM), which every call would
+// otherwise pay.  One call at a time uses the cache; a call that finds it in
+// use builds its own context.  The slab is released by bt_release_cache or
+// when another allocation of the library runs out of HBM
+// (release_cached_memory).  The cache is never destroyed: its threads and
+// device memory last until the process ends.
+struct BatchContext {
+  std::unique_ptr<RoundEngine> e0, e1;
+  void* slab = nullptr;
+  size_t slab_bytes = 0;
+  int init(int prec) {
+    if (e0) return BT_OK;
+    std::unique_ptr<RoundEngine> a(new RoundEngine(prec)), b(new RoundEngine(prec));
+    int rc = a->init();
+    if (rc) return rc;
+    rc = b->init(a->stream());
+    if (rc) return rc;
+    e0 = std::move(a);
+    e1 = std::move(b);
+    return BT_OK;
+  }
+  void free_slab() {
+    if (slab) {
+      if (e0) (void)hipStreamSynchronize(e0->stream());
+      (void)hipFree(slab);
+    }
+    slab = nullptr;
+    slab_bytes = 0;
+  }
+};
+
+struct BatchCache {
+  std::mutex mu;  // held for the whole of a bt_run_batch call that uses the cache
+  BatchContext ctx[2];
+};
+
+static BatchCache& batch_cache() {
+  static BatchCache* c = new BatchCache();  // never destroyed (see BatchContext)
+  return *c;
+}
+
+// Free the cached batch slabs unless a batch is running; returns the bytes
+// released.
+size_t release_cached_memory() {
+  BatchCache& bc = batch_cache();
+  std::unique_lock<std::mutex> lk(bc.mu, std::try_to_lock);
+  if (!lk.owns_lock()) return 0;
+  size_t n = 0;
+  for (BatchContext& x : bc.ctx) {
+    n += x.slab_bytes;
+    x.free_slab();
+  }
+  return n;
+}
+
+}  // namespace bt
+
 using namespace bt;
+
+extern "C" int bt_release_cache(void) {
+  (void)release_cached_memory();
+  return BT_OK;
+}
 
 // The lockstep-rounds batch; samples (nullable): one basis index per
 // trajectory drawn from its final state with u(seed, traj, draws used).
@@ -538,30 +611,36 @@ static int run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64
   if (!c) return fail(BT_ERR_ARG, "null circuit");
   if (ntraj <= 0 || !outcomes || !nouts || outcap < 0) return fail(BT_ERR_ARG, "bad batch arguments");
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
+  int rc = ensure_device();
+  if (rc) return rc;
   // two cohorts whose rounds alternate on one stream: while the GPU runs one
-  // cohort's pass, the host plans the other's next pass
-  RoundEngine eng0(prec), eng1(prec);
-  int rc = eng0.init();
+  // cohort's pass, the host plans the other's next pass; engines and slab
+  // from the cache when it is free, else built for this call
+  BatchCache& bc = batch_cache();
+  std::unique_lock<std::mutex> cache_lock(bc.mu, std::try_to_lock);
+  BatchContext own;
+  BatchContext& X = cache_lock.owns_lock() ? bc.ctx[prec == BT_C64 ? 0 : 1] : own;
+  rc = X.init(prec);
   if (rc) return rc;
-  rc = eng1.init(eng0.stream());
-  if (rc) return rc;
+  RoundEngine& eng0 = *X.e0;
+  RoundEngine& eng1 = *X.e1;
+  eng0.reset_counters();
+  eng1.reset_counters();
   RoundEngine* E[2] = {&eng0, &eng1};
   hipStream_t stream = eng0.stream();
   auto circ = std::shared_ptr<const btc::CircuitHost>(&c->h, [](const btc::CircuitHost*) {});
   const int nbits = c->h.n;
   const size_t bytes = (size_t)(1ull << nbits) * eng0.elem();
-  size_t freeb = 0, totalb = 0;
-  BT_HIP(hipMemGetInfo(&freeb, &totalb));
-  // resident states: all of them when they fit in 3/4 of free HBM
-  int64_t nslots = std::min<int64_t>(ntraj, (int64_t)(freeb / 4 * 3 / (bytes + 4096)));
-  if (nslots < 1) return fail(BT_ERR_NOMEM, "bt_run_batch: a single state does not fit in HBM");
-  std::vector<void*> slot_mem;
-  double* d_tmp = nullptr;
-  auto cleanup = [&]() {
-    for (void* p : slot_mem) (void)hipFree(p);
-    if (d_tmp) (void)hipFree(d_tmp);
-  };
-  {
+  int64_t nslots = 0;
+  if (X.slab && X.slab_bytes >= bytes * (size_t)ntraj) {
+    nslots = ntraj;  // the cached slab holds every state
+  } else {
+    X.free_slab();
+    size_t freeb = 0, totalb = 0;
+    BT_HIP(hipMemGetInfo(&freeb, &totalb));
+    // resident states: all of them when they fit in 3/4 of free HBM
+    nslots = std::min<int64_t>(ntraj, (int64_t)(freeb / 4 * 3 / (bytes + 4096)));
+    if (nslots < 1) return fail(BT_ERR_NOMEM, "bt_run_batch: a single state does not fit in HBM");
     // one allocation for all slots
     void* p = nullptr;
     while (nslots >= 1 && hipMalloc(&p, bytes * nslots) != hipSuccess) {
@@ -569,8 +648,19 @@ static int run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64
       nslots /= 2;
     }
     if (nslots < 1) return fail(BT_ERR_NOMEM, "bt_run_batch: state allocation failed");
-    slot_mem.push_back(p);
+    X.slab = p;
+    X.slab_bytes = bytes * (size_t)nslots;
   }
+  std::vector<void*> slot_mem;  // per-call buffers, freed at the end
+  double* d_tmp = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : slot_mem) (void)hipFree(p);
+    if (d_tmp) (void)hipFree(d_tmp);
+    if (&X == &own) {
+      for (int ch = 0; ch < 2; ch++) (void)E[ch]->finish_round();
+      own.free_slab();
+    }
+  };
   if (final_states && hipMalloc((void**)&d_tmp, sizeof(double) * 2 * (1ull << nbits)) != hipSuccess) {
     cleanup();
     return fail(BT_ERR_NOMEM, "bt_run_batch: export buffer");
@@ -612,7 +702,7 @@ static int run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64
       j->prec = prec;
       j->seed = seed;
       j->traj = (uint64_t)(traj0 + next);
-      j->state = (char*)slot_mem[0] + bytes * s;
+      j->state = (char*)X.slab + bytes * s;
       j->bytes = bytes;
       slot_of[next] = s;
       E[ch]->start_job(j);
@@ -808,7 +898,10 @@ static void* take_buffer(bt_server_s* S, size_t bytes) {
   void* p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) {
     (void)hipGetLastError();
-    return nullptr;
+    if (!release_cached_memory() || hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
   }
   return p;
 }
@@ -949,6 +1042,7 @@ extern "C" int bt_server_create(uint64_t hbm_budget_bytes, int max_slots, bt_ser
   int rc = ensure_device();
   if (rc) return rc;
   size_t freeb = 0, totalb = 0;
+  if (!hbm_budget_bytes) (void)release_cached_memory();  // the default budget is measured against free HBM
   BT_HIP(hipMemGetInfo(&freeb, &totalb));
   auto* S = new bt_server_s();
   S->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(freeb / 4 * 3);

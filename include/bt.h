@@ -215,6 +215,11 @@ int bt_run_batch_sample(bt_circuit c, int prec, uint64_t seed, int64_t traj0, in
 /* The same contract in ONE persistent launch: each workgroup owns a
  * trajectory, runs its interpreter on the device (lane 0) and takes the next
  * trajectory from an atomic counter. */
+/* bt_run_batch keeps its worker threads, stream and the slab of resident
+ * states between calls (a repeated batch pays no set-up); this frees the
+ * cached slab.  Other allocations of the library that run out of HBM free it
+ * too.  No counterpart in the reference (which allocates nothing on a device). */
+int bt_release_cache(void);
 int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
                             int64_t outcap, int64_t* nouts, double* final_states, bt_run_stats* stats);
 

@@ -26,8 +26,11 @@ ROCM_INC = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "include")
 def emu():
     if not os.path.exists(EMU_SO) or any(os.path.getmtime(s) > os.path.getmtime(EMU_SO) for s in SRCS):
         # host-only build: the HIP headers supply the vector types, no device code
+        # built under a per-process name, then renamed: parallel test workers may build at once
+        tmp = "%s.%d.tmp" % (EMU_SO, os.getpid())
         subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I" + ROCM_INC,
-                        "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-o", EMU_SO, SRCS[0]], check=True)
+                        "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-o", tmp, SRCS[0]], check=True)
+        os.replace(tmp, EMU_SO)
     L = ctypes.CDLL(EMU_SO)
     L.emu_error.restype = ctypes.c_char_p
     L.emu_set_exec(0)
