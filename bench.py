@@ -28,7 +28,8 @@ Beside it, on the same JSON line:
   small_configs the N=12 complex128 brickwork (time per circuit) and the N=8
                 noisy density matrix (complex64 vs complex128, trace, purity);
   serving       Poisson arrivals of mixed circuits through the continuous-
-                batching server: p50/p99 latency and throughput;
+                batching server: p50/p99 latency, queue wait and
+                throughput; then a second leg under a binding HBM budget;
   large_state   N=33 complex64 brickwork: 5-qubit blocks on FP32 MFMA vs the
                 register-tile executor vs unfused kernels, error and fidelity
                 against the complex128 run.
@@ -235,37 +236,53 @@ def small_configs_leg(bt, args):
     return out
 
 
-def serving_leg(bt, args):
+def serving_leg(bt, args, budget_gib=0.0, nreq=None, tag="serving"):
+    """Poisson arrivals of the serving workload through the continuous-
+    batching server.  Circuits: 128 distinct random ones (statevector N ~
+    U{10..28} or density matrix N ~ U{6..13}, depth ~ U{10..100}, brickwork
+    and noise as the batched workload), each request one of them with its own
+    trajectory.  The server's state pool is pre-sized for the sizes present
+    (Server.reserve, on this thread) before the clock starts.  budget_gib > 0:
+    the HBM budget binds and requests wait for admission (queue_ms)."""
     import numpy as np
     from bluetangle_jl_amd import workloads as W
     from bluetangle_jl_amd.circuit import Circuit, Server
-    rng = np.random.default_rng(args.serve_seed)
-    reqs = []
-    for i in range(args.serve_requests):
+    rng = np.random.default_rng(args.serve_seed + (1 if budget_gib else 0))
+    pool = []
+    for i in range(128):
         dm = rng.random() < 0.25
         N = int(rng.integers(6, 14)) if dm else int(rng.integers(10, 29))
         depth = int(rng.integers(10, 101))
         ops = W.brickwork_ops(N, depth, int(rng.integers(1 << 30)), ad_gamma=5e-3,
                               mz_prob=0.0 if dm else 0.1)
-        reqs.append((Circuit(ops, N, W.depolarizing_noise()), "dm" if dm else "sv", N, depth))
-    gaps = rng.exponential(1.0 / args.serve_rate, size=len(reqs))
-    srv = Server()
+        pool.append((Circuit(ops, N, W.depolarizing_noise()), "dm" if dm else "sv", N, depth))
+    n = nreq or args.serve_requests
+    pick = rng.integers(0, len(pool), size=n)
+    gaps = rng.exponential(1.0 / args.serve_rate, size=n)
+    budget = int(budget_gib * 2 ** 30)
+    srv = Server(hbm_budget_bytes=budget)
+    reserved = 0
+    for kind, N in sorted({(k, N) for _, k, N, _ in pool}):
+        reserved += srv.reserve(kind, "c64", N, 1 if budget else 2)
     ids = {}
+    peak = [0]
     t_start = time.perf_counter()
 
     def producer():
         t = t_start
-        for i, (c, kind, N, depth) in enumerate(reqs):
+        for i in range(n):
+            c, kind, N, depth = pool[pick[i]]
             t += gaps[i]
             d = t - time.perf_counter()
             if d > 0:
                 time.sleep(d)
             ids[srv.submit(c, kind, "c64", seed=11, traj=i)] = i
+            peak[0] = max(peak[0], srv.stats()["hbm_in_use"])
 
     th = threading.Thread(target=producer)
     th.start()
     done = {}
-    while len(done) < len(reqs):
+    while len(done) < n:
         got = srv.poll(wait=True)
         if not got:
             time.sleep(0.001)
@@ -276,13 +293,21 @@ def serving_leg(bt, args):
     th.join()
     srv.close()
     lat = np.array([r["latency_ms"] for r in done.values()])
+    qms = np.array([r["queue_ms"] for r in done.values()])
     bad = sum(1 for r in done.values() if r["status"] != 0)
-    return dict(requests=len(reqs), failed=bad, rate_per_s=args.serve_rate,
-                p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
-                throughput_rps=round(len(reqs) / (t_end - t_start), 2),
-                workload="Poisson arrivals; statevector N~U{10..28} (75%) or density matrix N~U{6..13} (25%), "
-                         "brickwork depth~U{10..100}, depolarizing 1e-3/1e-2 + amplitude damping 5e-3, "
-                         "MZ p=0.1 per qubit per layer (statevector); complex64")
+    out = dict(requests=n, failed=bad, rate_per_s=args.serve_rate,
+               p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
+               queue_p50_ms=round(float(np.percentile(qms, 50)), 2),
+               queue_p99_ms=round(float(np.percentile(qms, 99)), 2),
+               throughput_rps=round(n / (t_end - t_start), 2), pool_buffers_reserved=int(reserved),
+               workload="Poisson arrivals; statevector N~U{10..28} (75%) or density matrix N~U{6..13} (25%), "
+                        "brickwork depth~U{10..100}, depolarizing 1e-3/1e-2 + amplitude damping 5e-3, "
+                        "MZ p=0.1 per qubit per layer (statevector); complex64; 128 distinct circuits, one "
+                        "trajectory per request")
+    if budget:
+        out["hbm_budget_gib"] = budget_gib
+        out["peak_hbm_in_use_gib"] = round(peak[0] / 2 ** 30, 3)
+    return out
 
 
 def c5_leg(bt, args):
@@ -474,8 +499,10 @@ def main():
     ap.add_argument("--plain-traj", type=int, default=64)
     ap.add_argument("--per-op-traj", type=int, default=4)
     ap.add_argument("--gate-N", type=int, default=30)
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-rate", type=float, default=24.0)
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=16.0, help="HBM budget of the second serving leg (0: off)")
+    ap.add_argument("--serve-budget-requests", type=int, default=300)
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-reference", type=int, default=1,
@@ -559,6 +586,10 @@ def main():
     if "serve" not in skip:
         out["serving"] = serving_leg(bt, args)
         log("serving", out["serving"])
+        if args.serve_budget_gib > 0:
+            out["serving"]["budget_leg"] = serving_leg(bt, args, budget_gib=args.serve_budget_gib,
+                                                       nreq=args.serve_budget_requests)
+            log("serving (budget)", out["serving"]["budget_leg"])
     if args.c5 and "c5" not in skip:
         out["large_state"] = c5_leg(bt, args)
         log("c5", out["large_state"])

@@ -33,6 +33,7 @@ SYMBOLS = [
     "bt_run_batch", "bt_run_batch_persistent", "bt_release_cache", "bt_set_tile_bits", "bt_set_tuning", "bt_get_tuning",
     "bt_autotune_batch",
     "bt_server_create", "bt_server_submit", "bt_server_poll", "bt_server_fetch", "bt_server_stats",
+    "bt_server_reserve",
     "bt_server_destroy",
 ]
 
@@ -152,6 +153,7 @@ def lib():
             "bt_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(Result), ctypes.c_int, ctypes.c_int, ip]),
             "bt_server_fetch": (ctypes.c_int, [vp, ctypes.c_int64, ip, ctypes.c_int64, i64p, dp, ctypes.c_uint64]),
             "bt_server_stats": (ctypes.c_int, [vp, ip, ip, u64p]),
+            "bt_server_reserve": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]),
             "bt_server_destroy": (ctypes.c_int, [vp]),
             "bt_release_cache": (ctypes.c_int, []),
         }

@@ -465,6 +465,18 @@ class Server:
         self._circ.pop(rid, None)
         return outs[: min(outcap, nout.value)].tolist(), p
 
+    def reserve(self, kind="sv", prec="c64", nqubits=1, count=1) -> int:
+        """Pre-size the state pool on the calling thread (count buffers for
+        requests of this kind, precision and qubit count, within the HBM
+        budget), so admitting them allocates nothing on the serving thread.
+        Returns the number of buffers added."""
+        from .core import PRECISIONS
+        kinds = {"sv": 0, "dm": 1}
+        n = ctypes.c_int()
+        L.check(L.lib().bt_server_reserve(self.handle, kinds[kind], PRECISIONS[prec], int(nqubits), int(count),
+                                          ctypes.byref(n)))
+        return n.value
+
     def stats(self):
         a, q = ctypes.c_int(), ctypes.c_int()
         u = ctypes.c_uint64()

@@ -43,11 +43,16 @@ int effective_tpb(int setting, uint64_t max_tiles, uint64_t total_tiles);
 struct RoundItem {
   void* state;
   uint64_t table;     // byte offset of the job's table in the round's blob
+  uint64_t part0;     // first per-tile reduction slot of this item
   uint32_t ntiles;    // tiles of the pass
   uint32_t block0;    // first workgroup of this item
   uint32_t nblocks;
   int32_t fresh;      // state is |0..0> and not yet written
 };
+
+// reduction slots per tile: one per wavefront of the workgroup
+template <typename R>
+constexpr int kSlotsPerTile = TileCfg<R>::NT / 64;
 
 // table layout: PassGeom, then int32 nmats (padded to 16 B), then nmats matrix entries
 constexpr size_t kGeomBytes = (sizeof(PassGeom) + 15) / 16 * 16;
@@ -55,10 +60,13 @@ constexpr size_t kGeomBytes = (sizeof(PassGeom) + 15) / 16 * 16;
 
 // One workgroup per `tpb` consecutive tiles of one item.  nb_uniform != 0:
 // every item has that many workgroups (the block -> item map is a division),
-// else a binary search over the items' first blocks.
+// else a binary search over the items' first blocks.  Reductions go to one
+// slot per tile and wavefront (part[j * nslots + slot]), so an item's sums do
+// not depend on tpb or on the other items of the launch.
 template <typename R>
 __global__ void __launch_bounds__(TileCfg<R>::NT, TileCfg<R>::WAVES) k_round(const RoundItem* items, int nitems, uint32_t nb_uniform,
-                                                          const unsigned char* blob, int tpb, double* part) {
+                                                          const unsigned char* blob, int tpb, double* part,
+                                                          uint64_t nslots) {
   constexpr int NT = TileCfg<R>::NT;
   extern __shared__ __align__(16) unsigned char smem[];
   C<R>* tile = (C<R>*)smem;
@@ -96,16 +104,17 @@ __global__ void __launch_bounds__(TileCfg<R>::NT, TileCfg<R>::WAVES) k_round(con
   const uint32_t local = blockIdx.x - it.block0;
   const uint64_t t0 = (uint64_t)local * tpb;
   const uint64_t tend = std::min<uint64_t>((uint64_t)it.ntiles, t0 + tpb);
-  tile_pass<R, uint32_t>((C<R>*)it.state, geom, mats, tile, t0, 1, tend, red, sums, it.fresh);
-  if (threadIdx.x < kAcc) part[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = sums[threadIdx.x];
+  tile_pass<R, uint32_t>((C<R>*)it.state, geom, mats, tile, t0, 1, tend, red, sums, it.fresh, part, nslots,
+                         it.part0);
 }
 
-// out[item, j]: fixed-order sum of the item's partials (grid: items x nacc,
-// the accumulators the round's reductions use)
+// out[item, j]: fixed-order sum of the item's per-tile slots (grid: items x
+// nacc, the accumulators the round's reductions use)
 __global__ void __launch_bounds__(256) k_round_finalize(const RoundItem* items, const double* part, uint64_t stride,
-                                                        double* out) {
+                                                        int slots_per_tile, double* out) {
   const RoundItem it = items[blockIdx.x];
-  const double t = sum_blocks(part + (uint64_t)blockIdx.y * stride + it.block0, it.nblocks);
+  const double t = sum_blocks(part + (uint64_t)blockIdx.y * stride + it.part0,
+                              (uint32_t)((uint64_t)it.ntiles * (uint64_t)slots_per_tile));
   if (threadIdx.x == 0) out[(uint64_t)blockIdx.x * gridDim.y + blockIdx.y] = t;
 }
 
@@ -371,6 +380,8 @@ class RoundEngine {
     const int tpb = effective_tpb(tiles_per_block(), max_tiles, total_tiles);
     size_t blob = 0;
     uint32_t blocks = 0;
+    uint64_t nslots = 0;
+    const int spt = prec_ == BT_C64 ? kSlotsPerTile<float> : kSlotsPerTile<double>;
     std::vector<RoundItem> items(act.size());
     size_t maxT = 0;
     uint32_t nb_uniform = 0;
@@ -382,6 +393,8 @@ class RoundEngine {
       it.state = j->state;
       it.table = blob;
       it.ntiles = (uint32_t)(1ull << (j->nbits - T));
+      it.part0 = nslots;
+      nslots += (uint64_t)it.ntiles * (uint64_t)spt;
       it.block0 = blocks;
       it.nblocks = (it.ntiles + tpb - 1) / tpb;
       it.fresh = j->fresh;
@@ -390,7 +403,10 @@ class RoundEngine {
       blocks += it.nblocks;
       blob += (j->table.size() + 15) / 16 * 16;
     }
-    int rc = reserve(blob, act.size(), blocks);
+    nacc_ = kAccSmall;
+    for (Job* j : act)
+      if (j->plan.red == bti::RED_RHO && j->plan.red_k == 3) nacc_ = kAcc;
+    int rc = reserve(blob, act.size(), nslots * (uint64_t)nacc_);
     if (rc) return rc;
     pool_.parallel_for((int)act.size(), [&](int i) {
       memcpy((unsigned char*)h_blob_ + items[i].table, act[i]->table.data(), act[i]->table.size());
@@ -404,19 +420,16 @@ class RoundEngine {
     if (prec_ == BT_C64) {
       (void)hipFuncSetAttribute((const void*)k_round<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL((k_round<float>), dim3(blocks), dim3(TileCfg<float>::NT), lds, stream_, d_items_,
-                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, tpb, d_part_);
+                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, tpb, d_part_, nslots);
     } else {
       (void)hipFuncSetAttribute((const void*)k_round<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL((k_round<double>), dim3(blocks), dim3(TileCfg<double>::NT), lds, stream_, d_items_,
-                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, tpb, d_part_);
+                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, tpb, d_part_, nslots);
     }
     BT_HIP(hipGetLastError());
     BT_HIP(hipEventRecord(ev_b_, stream_));
-    nacc_ = kAccSmall;
-    for (Job* j : act)
-      if (j->plan.red == bti::RED_RHO && j->plan.red_k == 3) nacc_ = kAcc;
     hipLaunchKernelGGL(k_round_finalize, dim3((unsigned)items.size(), (unsigned)nacc_), dim3(256), 0, stream_,
-                       d_items_, d_part_, (uint64_t)blocks, d_out_);
+                       d_items_, d_part_, nslots, spt, d_out_);
     BT_HIP(hipGetLastError());
     BT_HIP(hipMemcpyAsync(h_out_, d_out_, sizeof(double) * nacc_ * items.size(), hipMemcpyDeviceToHost, stream_));
     BT_HIP(hipEventRecord(ev_done_, stream_));
@@ -460,7 +473,8 @@ class RoundEngine {
   }
 
  private:
-  int reserve(size_t blob, size_t nitems, uint32_t blocks) {
+  // pass tables (bytes), items, reduction slots (doubles)
+  int reserve(size_t blob, size_t nitems, uint64_t part_doubles) {
     if (blob > cap_blob_) {
       if (h_blob_) (void)hipHostFree(h_blob_);
       if (d_blob_) (void)hipFree(d_blob_);
@@ -480,10 +494,10 @@ class RoundEngine {
       BT_HIP(hipMalloc((void**)&d_out_, sizeof(double) * kAcc * cap_items_));
       BT_HIP(hipHostMalloc((void**)&h_out_, sizeof(double) * kAcc * cap_items_));
     }
-    if (blocks > cap_blocks_) {
+    if (part_doubles > cap_part_) {
       if (d_part_) (void)hipFree(d_part_);
-      cap_blocks_ = std::max<size_t>(blocks, cap_blocks_ * 2);
-      BT_HIP(hipMalloc((void**)&d_part_, sizeof(double) * kAcc * cap_blocks_));
+      cap_part_ = std::max<size_t>(part_doubles, cap_part_ * 2);
+      BT_HIP(hipMalloc((void**)&d_part_, sizeof(double) * cap_part_));
     }
     return BT_OK;
   }
@@ -502,7 +516,7 @@ class RoundEngine {
   RoundItem* d_items_ = nullptr;
   size_t cap_items_ = 0;
   double* d_part_ = nullptr;
-  size_t cap_blocks_ = 0;
+  size_t cap_part_ = 0;
   double* d_out_ = nullptr;
   double* h_out_ = nullptr;
   int64_t rounds_ = 0;
@@ -864,7 +878,7 @@ struct bt_server_s {
   std::unordered_map<size_t, std::vector<void*>> free_bufs;
   std::vector<void*> to_release;  // freed by the worker (it owns allocation)
   std::unordered_map<uint64_t, std::shared_ptr<const btc::CircuitHost>> sv_cache, dm_cache;
-  std::unique_ptr<bt::RoundEngine> eng[2];
+  std::unique_ptr<bt::RoundEngine> eng[2][2];  // [precision][cohort]; a precision's cohorts share one stream
   hipStream_t fetch_stream = nullptr;
   double* d_probs = nullptr;
   size_t cap_probs = 0;
@@ -922,46 +936,85 @@ static void finish_request(bt_server_s* S, Request* r, int err) {
   (void)S;
 }
 
+// The worker: requests are split into two cohorts whose rounds alternate on
+// each precision's stream, as in bt_run_batch: while the device runs one
+// cohort's pass, the host plans the other's next pass (interpreters and pass
+// tables on the engine's thread pool).  Each cycle, for cohort c: wait for
+// its last round, retire what finished, admit queued requests FIFO while
+// they fit the HBM budget and the slot cap (into cohort c), and launch its
+// next round (one launch per precision over the cohort's requests).
 static void server_loop(bt_server_s* S) {
   (void)hipSetDevice(S->device);
-  std::vector<std::shared_ptr<Request>> running;
+  std::vector<std::shared_ptr<Request>> running[2];  // per cohort
+  auto engine = [&](int prec, int c) -> RoundEngine* {
+    const int e = prec == BT_C64 ? 0 : 1;
+    if (!S->eng[e][0]) {
+      std::unique_ptr<RoundEngine> a(new RoundEngine(prec)), b(new RoundEngine(prec));
+      if (a->init() || b->init(a->stream())) return nullptr;
+      S->eng[e][0] = std::move(a);
+      S->eng[e][1] = std::move(b);
+    }
+    return S->eng[e][c].get();
+  };
+  auto reject = [&](const std::shared_ptr<Request>& r, int err) {
+    r->t_admit = now_ms();
+    finish_request(S, r.get(), err);
+    S->finished[r->id] = r;
+    S->unpolled.push_back(r->res);
+    S->cv_done.notify_all();
+  };
   for (;;) {
-    {
+    bool idle = true;
+    for (int c = 0; c < 2; c++) {
+      // 1. the cohort's launched round: wait, hand back the reductions
+      for (int e = 0; e < 2; e++)
+        for (int cc = 0; cc < 2; cc++)
+          if (cc == c && S->eng[e][cc]) {
+            const int rc = S->eng[e][cc]->finish_round();
+            if (rc)
+              for (auto& r : running[c])
+                if ((r->prec == BT_C64 ? 0 : 1) == e && !r->job->done) {
+                  r->job->done = 1;
+                  if (!r->job->err) r->job->err = rc;
+                }
+          }
       std::unique_lock<std::mutex> lk(S->mu);
+      // 2. retire
+      std::vector<std::shared_ptr<Request>> still;
+      for (auto& r : running[c]) {
+        if (!r->job->done) {
+          still.push_back(r);
+          continue;
+        }
+        finish_request(S, r.get(), r->job->err);
+        r->job->ip.reset();
+        S->finished[r->id] = r;
+        S->unpolled.push_back(r->res);
+      }
+      if (still.size() != running[c].size()) S->cv_done.notify_all();
+      running[c].swap(still);
       for (void* p : S->to_release) (void)hipFree(p);
       S->to_release.clear();
-      // admission, FIFO
-      while (!S->queue.empty() && (int)running.size() < S->max_slots) {
+      // 3. admission, FIFO, into this cohort
+      while (!S->stop && !S->queue.empty() && (int)(running[0].size() + running[1].size()) < S->max_slots) {
         std::shared_ptr<Request> r = S->queue.front();
         if (r->bytes > S->budget) {  // can never fit
           S->queue.pop_front();
-          r->t_admit = now_ms();
-          finish_request(S, r.get(), BT_ERR_NOMEM);
-          S->finished[r->id] = r;
-          S->unpolled.push_back(r->res);
-          S->cv_done.notify_all();
+          reject(r, BT_ERR_NOMEM);
           continue;
         }
         if (S->in_use + r->bytes > S->budget) break;
         void* mem = take_buffer(S, r->bytes);
         if (!mem) {
-          if (!running.empty()) break;  // retry when a resident request retires
-          S->queue.pop_front();         // nothing resident: HBM is really short
-          r->t_admit = now_ms();
-          finish_request(S, r.get(), BT_ERR_NOMEM);
-          S->finished[r->id] = r;
-          S->unpolled.push_back(r->res);
-          S->cv_done.notify_all();
+          if (!running[0].empty() || !running[1].empty()) break;  // retry when a resident request retires
+          S->queue.pop_front();  // nothing resident: HBM is really short
+          reject(r, BT_ERR_NOMEM);
           continue;
         }
         S->queue.pop_front();
         S->in_use += r->bytes;
         r->t_admit = now_ms();
-        const int e = r->prec == BT_C64 ? 0 : 1;
-        if (!S->eng[e]) {
-          S->eng[e].reset(new RoundEngine(r->prec));
-          if (S->eng[e]->init()) S->eng[e].reset();
-        }
+        RoundEngine* E = engine(r->prec, c);
         r->job.reset(new Job());
         Job* j = r->job.get();
         j->id = r->id;
@@ -973,64 +1026,60 @@ static void server_loop(bt_server_s* S) {
         j->traj = r->traj;
         j->state = mem;
         j->bytes = r->bytes;
-        if (!S->eng[e]) {
+        if (!E) {
           j->done = 1;
           j->err = BT_ERR_HIP;
         } else {
-          S->eng[e]->start_job(j);
+          E->start_job(j);
         }
-        running.push_back(r);
+        running[c].push_back(r);
       }
-      S->nrunning = (int)running.size();
-      if (running.empty()) {
+      S->nrunning = (int)(running[0].size() + running[1].size());
+      if (S->stop) {
+        lk.unlock();
+        for (int e = 0; e < 2; e++)
+          for (int cc = 0; cc < 2; cc++)
+            if (S->eng[e][cc]) (void)S->eng[e][cc]->finish_round();
+        lk.lock();
+        for (int cc = 0; cc < 2; cc++) {
+          for (auto& r : running[cc]) {
+            S->free_bufs[r->bytes].push_back(r->job->state);
+            S->cached += r->bytes;
+            S->in_use -= r->bytes;
+          }
+          running[cc].clear();
+        }
+        S->nrunning = 0;
+        return;
+      }
+      lk.unlock();
+      // 4. the cohort's next round, one launch per precision
+      for (int e = 0; e < 2; e++) {
+        if (!S->eng[e][c]) continue;
+        std::vector<Job*> jobs;
+        for (auto& r : running[c])
+          if ((r->prec == BT_C64 ? 0 : 1) == e && !r->job->done) jobs.push_back(r->job.get());
+        if (jobs.empty()) continue;
+        idle = false;
+        const int rc = S->eng[e][c]->launch_round(jobs);
+        if (rc)
+          for (Job* j : jobs) {
+            j->done = 1;
+            if (!j->err) j->err = rc;
+          }
+      }
+      if (!running[c].empty()) idle = false;
+    }
+    if (idle) {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (running[0].empty() && running[1].empty()) {
         if (S->stop) return;
         S->cv_work.wait(lk, [&] {
           return S->stop || (!S->queue.empty() &&
                              (S->queue.front()->bytes > S->budget || S->in_use + S->queue.front()->bytes <= S->budget));
         });
-        continue;
-      }
-      if (S->stop) {
-        for (auto& r : running) {
-          S->free_bufs[r->bytes].push_back(r->job->state);
-          S->cached += r->bytes;
-          S->in_use -= r->bytes;
-        }
-        running.clear();
-        S->nrunning = 0;
-        return;
       }
     }
-    // one round per precision over the resident requests
-    for (int e = 0; e < 2; e++) {
-      if (!S->eng[e]) continue;
-      std::vector<Job*> jobs;
-      for (auto& r : running)
-        if ((r->prec == BT_C64 ? 0 : 1) == e && !r->job->done) jobs.push_back(r->job.get());
-      if (jobs.empty()) continue;
-      const int rc = S->eng[e]->round(jobs);
-      if (rc)
-        for (Job* j : jobs) {
-          j->done = 1;
-          if (!j->err) j->err = rc;
-        }
-    }
-    // retire
-    std::lock_guard<std::mutex> lk(S->mu);
-    std::vector<std::shared_ptr<Request>> still;
-    for (auto& r : running) {
-      if (!r->job->done) {
-        still.push_back(r);
-        continue;
-      }
-      finish_request(S, r.get(), r->job->err);
-      r->job->ip.reset();
-      S->finished[r->id] = r;
-      S->unpolled.push_back(r->res);
-    }
-    if (still.size() != running.size()) S->cv_done.notify_all();
-    running.swap(still);
-    S->nrunning = (int)running.size();
   }
 }
 
@@ -1107,6 +1156,36 @@ extern "C" int bt_server_poll(bt_server S, bt_result* results, int cap, int wait
     S->unpolled.pop_front();
   }
   *n = k;
+  return BT_OK;
+}
+
+extern "C" int bt_server_reserve(bt_server S, int kind, int prec, int nqubits, int count, int* added) {
+  if (!S) return fail(BT_ERR_ARG, "null server");
+  if (kind != BT_SV && kind != BT_DM) return fail(BT_ERR_ARG, "bad state kind");
+  if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
+  const int nbits = kind == BT_DM ? 2 * nqubits : nqubits;
+  if (nqubits < 1 || nbits > 36 || count < 0) return fail(BT_ERR_ARG, "bt_server_reserve: bad size or count");
+  const size_t bytes = state_bytes(kind, prec, nqubits);
+  int n = 0;
+  (void)hipSetDevice(S->device);
+  for (; n < count; n++) {
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (S->in_use + S->cached + bytes > S->budget) break;
+      S->cached += bytes;  // counted now so that concurrent reservations respect the budget
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->cached -= bytes;
+      break;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->free_bufs[bytes].push_back(p);
+  }
+  if (added) *added = n;
+  S->cv_work.notify_all();
   return BT_OK;
 }
 
@@ -1191,7 +1270,8 @@ extern "C" int bt_server_destroy(bt_server S) {
   for (auto& kv : S->free_bufs)
     for (void* p : kv.second) (void)hipFree(p);
   for (void* p : S->to_release) (void)hipFree(p);
-  for (int e = 0; e < 2; e++) S->eng[e].reset();
+  for (int e = 0; e < 2; e++)
+    for (int c = 1; c >= 0; c--) S->eng[e][c].reset();  // cohort 1 uses cohort 0's stream
   if (S->d_probs) (void)hipFree(S->d_probs);
   if (S->fetch_stream) (void)hipStreamDestroy(S->fetch_stream);
   delete S;

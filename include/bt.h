@@ -288,6 +288,12 @@ int bt_server_poll(bt_server srv, bt_result* results, int cap, int wait, int* n)
 int bt_server_fetch(bt_server srv, int64_t id, int* outcomes, int64_t outcap, int64_t* nout, double* probs,
                     uint64_t nprobs);
 int bt_server_stats(bt_server srv, int* active, int* queued, uint64_t* hbm_in_use);
+/* Pre-size the state pool: allocate `count` buffers for requests of this
+ * kind, precision and qubit count on the calling thread, so admitting such
+ * requests later allocates nothing on the serving thread.  Stops at the HBM
+ * budget; *added (nullable) gets the number of buffers added.  No counterpart
+ * in the reference (which has no serving loop). */
+int bt_server_reserve(bt_server srv, int kind, int prec, int nqubits, int count, int* added);
 int bt_server_destroy(bt_server srv);
 
 #ifdef __cplusplus

@@ -192,3 +192,55 @@ def test_arrivals_during_execution(bt):
         outs, probs = s1.fetch(r1)
         s1.close()
         assert outs == results[rid][0] and np.array_equal(probs, results[rid][1]), (c.N, traj)
+
+
+def test_mix_equals_alone_whatever_the_grouping(bt):
+    """A request's reductions are summed per tile in a fixed order, so its
+    result does not depend on how many tiles a workgroup takes, which the
+    engine sets from the whole launch: an N=18 request alone (one tile per
+    workgroup) and beside an N=26 request (several per workgroup) gives the
+    same outcomes and probabilities bit for bit."""
+    from bluetangle_jl_amd import _lib
+    from bluetangle_jl_amd.circuit import Circuit, Server
+    from bluetangle_jl_amd import workloads as W
+    nm = W.depolarizing_noise()
+    mid = Circuit(W.brickwork_ops(18, 16, 61, ad_gamma=5e-3, mz_prob=0.2), 18, nm)
+    big = Circuit(W.brickwork_ops(26, 4, 62, ad_gamma=5e-3, mz_prob=0.1), 26, nm)
+    try:
+        _lib.check(_lib.lib().bt_set_tuning(0, 0, 0, 32))
+        res = {}
+        for mix in (False, True):
+            srv = Server()
+            ids = [srv.submit(mid, "sv", "c64", 4, t) for t in range(3)]
+            if mix:
+                srv.submit(big, "sv", "c64", 4, 0)
+            done = _drain(srv, len(ids) + (1 if mix else 0))
+            assert all(done[i]["status"] == 0 for i in ids)
+            res[mix] = [srv.fetch(i) for i in ids]
+            srv.close()
+        for (oa, pa), (ob, pb) in zip(res[False], res[True]):
+            assert oa == ob and np.array_equal(pa, pb)
+    finally:
+        _lib.check(_lib.lib().bt_set_tuning(0, 0, 0, 0))
+
+
+def test_reserve_presizes_the_pool(bt):
+    """Server.reserve allocates state buffers on the caller's thread within
+    the budget; requests of that size then run from the pool."""
+    from bluetangle_jl_amd.circuit import Circuit, Server
+    from bluetangle_jl_amd import workloads as W
+    N = 16
+    state = (1 << N) * 8
+    c = Circuit(W.brickwork_ops(N, 6, 71, ad_gamma=5e-3, mz_prob=0.1), N, W.depolarizing_noise())
+    srv = Server(hbm_budget_bytes=3 * state)
+    assert srv.reserve("sv", "c64", N, 5) == 3  # the budget holds three
+    ids = [srv.submit(c, "sv", "c64", 2, t) for t in range(6)]
+    got = {}
+    while len(got) < len(ids):  # fetch as they finish: unfetched states hold the budget
+        for x in srv.poll(wait=True):
+            assert x["status"] == 0
+            got[x["id"]] = srv.fetch(x["id"], probs=False)[0]
+    outs = [got[i] for i in ids]
+    srv.close()
+    ref, _, _ = c.run_batch(6, seed=2, prec="c64")
+    assert outs == ref
