@@ -146,12 +146,27 @@ def gate_layer_leg(bt, args):
     dt = time.perf_counter() - t0
     nlayers = reps * len(ops)
     gbs = nlayers * 2.0 * (1 << N) * 8 / dt / 1e9
+    # every target bit position on its own: 4 back-to-back launches per bit
+    per_bit = []
+    for b in range(N):
+        op = ops[N - 1 - b]  # qubit q acts on bit N - q
+        bt.apply(st, op)
+        st.sync()
+        t1 = time.perf_counter()
+        for _ in range(4):
+            bt.apply(st, op)
+        st.sync()
+        per_bit.append(4 * 2.0 * (1 << N) * 8 / (time.perf_counter() - t1) / 1e9)
     obs = observables_on(bt, st, N)
     del st
     return dict(N=N, dtype="complex64", layers=nlayers, ms_per_layer=round(dt / nlayers * 1e3, 4),
                 achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
+                per_bit_gbs=[round(x, 1) for x in per_bit], per_bit_min_frac=round(min(per_bit) / HBM_PEAK_GBS, 4),
+                per_bit_mean_frac=round(sum(per_bit) / len(per_bit) / HBM_PEAK_GBS, 4),
+                per_bit_min_bit=int(min(range(N), key=lambda i: per_bit[i])),
                 note="one Hadamard per layer on each qubit in turn; read+write of the state per layer; wall time "
-                     "of back-to-back launches", observables=obs)
+                     "of back-to-back launches; per_bit_gbs[b]: four launches on target bit b alone",
+                observables=obs)
 
 
 def observables_on(bt, st, N):

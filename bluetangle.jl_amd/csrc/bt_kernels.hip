@@ -55,6 +55,39 @@ __global__ void __launch_bounds__(256) k_apply(ApplyArgs<R, K> a) {
   }
 }
 
+// One-qubit operator on a low target bit b <= 6 of a complex64 state: every
+// lane takes two consecutive amplitudes with one 16-byte access, so a
+// wavefront reads and writes 1 KiB contiguous per instruction whatever the
+// target (the generic kernel above gives each lane one amplitude of the pair,
+// 8 bytes every 2^(b+1) x 8 bytes, for low b).  b = 0: the pair is in the
+// lane; b >= 1: the partner amplitudes are in lane ^ 2^(b-1) of the same
+// wavefront, exchanged with cross-lane shuffles.
+__global__ void __launch_bounds__(256) k_apply1_low(float4* __restrict__ s, uint64_t nvec, int b, float2 u00,
+                                                    float2 u01, float2 u10, float2 u11, float scale) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // every lane of a wavefront runs the same trips (nvec is a multiple of 64),
+  // so the shuffles below always find their partner lane active
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) {
+    const float4 v = s[i];
+    const float2 x0 = make_float2(v.x, v.y), x1 = make_float2(v.z, v.w);
+    float2 y0, y1;
+    if (b == 0) {
+      y0 = cfma(cmul(u00, x0), u01, x1);
+      y1 = cfma(cmul(u10, x0), u11, x1);
+    } else {
+      const int m = 1 << (b - 1);
+      const float2 p0 = make_float2(__shfl_xor(x0.x, m, 64), __shfl_xor(x0.y, m, 64));
+      const float2 p1 = make_float2(__shfl_xor(x1.x, m, 64), __shfl_xor(x1.y, m, 64));
+      // this lane holds the amplitudes with target bit = (i >> (b - 1)) & 1
+      const bool hi = (i >> (b - 1)) & 1;
+      const float2 cs = hi ? u11 : u00, co = hi ? u10 : u01;
+      y0 = cfma(cmul(cs, x0), co, p0);
+      y1 = cfma(cmul(cs, x1), co, p1);
+    }
+    s[i] = make_float4(y0.x * scale, y0.y * scale, y1.x * scale, y1.y * scale);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
@@ -399,6 +432,16 @@ static int apply_t(bt_state_s* s, int nbits, const double* m, const Placement& p
     BT_HIP(hipStreamSynchronize(s->stream));
     BT_HIP(hipMemcpy(s->d_mat, hm.data(), sizeof(C<R>) * D * D, hipMemcpyHostToDevice));
     a.mptr = (const C<R>*)s->d_mat;
+  }
+  if constexpr (K == 1 && sizeof(R) == 4) {
+    // uncontrolled one-qubit operator on a low bit: the coalesced kernel
+    if (p.ctrl == 0 && p.nfixed == 1 && p.fixed[0] <= 6 && nbits >= 7) {
+      const uint64_t nvec = 1ull << (nbits - 1);  // two amplitudes per lane
+      hipLaunchKernelGGL(k_apply1_low, dim3(grid_for(nvec)), dim3(256), 0, s->stream, (float4*)s->d, nvec,
+                         p.fixed[0], a.m[0], a.m[2], a.m[1], a.m[3], (float)scale);
+      BT_HIP(hipGetLastError());
+      return BT_OK;
+    }
   }
   hipLaunchKernelGGL((k_apply<R, K>), dim3(grid_for(a.ngroups)), dim3(256), 0, s->stream, a);
   BT_HIP(hipGetLastError());
