@@ -21,6 +21,8 @@ Beside it, on the same JSON line:
                 sample of that circuit and on the two small configurations;
   plain_loop    the same circuit one trajectory per call (bt_run_circuit: one
                 launch per pass, one circuit per launch);
+  persistent    the same batch as one persistent launch (device-side
+                interpreters), on a sample;
   per_op_loop   the plain form: apply(state, op; noise) op by op;
   gate_layer    single-gate layers on an N=30 complex64 state (per-op
                 kernels), achieved GB/s against 8 TB/s; then expect(state, "Z")
@@ -114,6 +116,18 @@ def plain_loop_leg(bt, c, args):
     for t in range(n):
         c.run_on(st.set_basis(0), seed=7, traj=t)
     return n / (time.perf_counter() - t0)
+
+
+def persistent_leg(bt, c, args):
+    """The batching form as one persistent launch (bt_run_batch_persistent:
+    each workgroup runs a trajectory's interpreter on its lane 0 and takes the
+    next trajectory from an atomic counter), on a sample of the same batch."""
+    n = args.persistent_traj
+    c.run_batch(64, seed=3, prec="c64", mode="persistent")  # warm
+    t0 = time.perf_counter()
+    _, _, st = c.run_batch(n, seed=1, prec="c64", mode="persistent")
+    dt = time.perf_counter() - t0
+    return n / dt, st
 
 
 def per_op_loop_leg(bt, cfg, args):
@@ -362,6 +376,13 @@ def c5_leg(bt, args):
     t0 = time.perf_counter()
     _, stile = c.run_on(a)
     tile_s = time.perf_counter() - t0
+    fusion = None
+    if args.c5_autotune:  # fusion width: max_k 3, 4, 5 (MFMA) or the register-tile executor, on a scratch state
+        k, ms = c.autotune_fused(a)
+        fusion = dict(best=("register-tile executor" if k == 0 else "fused blocks, max_k=%d" % k),
+                      ms={"k3": round(ms[0], 1), "k4": round(ms[1], 1), "k5_mfma": round(ms[2], 1),
+                          "register_tile": round(ms[3], 1)})
+        log("c5 fusion autotune", fusion)
     per_layer = len(ops) // layers
     a.set_basis(0)
     a.sync()
@@ -380,7 +401,7 @@ def c5_leg(bt, args):
                 max_amp_err_vs_fp64=err, fidelity_vs_fp64=fid, fp64_reference_s=round(ref_s, 3),
                 tile_executor_s=round(tile_s, 3), tile_executor_passes=int(stile["passes"]),
                 unfused_layer_s=round(unf_s, 3), unfused_gbs=round(unf_gbs, 1),
-                unfused_s_extrapolated=round(unf_s * layers, 2))
+                unfused_s_extrapolated=round(unf_s * layers, 2), fusion_autotune=fusion)
 
 
 def cpu_leg(args):
@@ -513,6 +534,7 @@ def main():
     ap.add_argument("--tuning", default="", help="T,C,B: fixed tile bits, coalescing bits, tiles per workgroup")
     ap.add_argument("--plain-traj", type=int, default=64)
     ap.add_argument("--per-op-traj", type=int, default=4)
+    ap.add_argument("--persistent-traj", type=int, default=1024)
     ap.add_argument("--gate-N", type=int, default=30)
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=40.0)
@@ -526,6 +548,7 @@ def main():
     ap.add_argument("--no-c5", dest="c5", action="store_false", help="skip the N=33 large-state leg")
     ap.add_argument("--c5-N", type=int, default=33)
     ap.add_argument("--c5-layers", type=int, default=40)
+    ap.add_argument("--c5-autotune", type=int, default=1, help="run the fusion-width autotuner on the C5 circuit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (the value is one GPU's "
@@ -587,6 +610,12 @@ def main():
                              "batched_speedup": round(value / pl, 2),
                              "executor": "bt_run_circuit per trajectory (fused tile passes, one launch per pass)"}
         log("plain loop %.2f traj/s" % pl)
+        pv, pst = persistent_leg(bt, c, args)
+        out["persistent"] = {"value": round(pv, 2), "unit": "trajectories/s", "sample": args.persistent_traj,
+                             "rounds_over_persistent": round(value / pv, 2),
+                             "passes_per_trajectory": round(pst["passes"] / args.persistent_traj, 3),
+                             "executor": "bt_run_batch_persistent: one launch, device-side interpreters"}
+        log("persistent %.2f traj/s" % pv)
         po = per_op_loop_leg(bt, cfg, args)
         out["per_op_loop"] = {"value": round(po, 4), "unit": "trajectories/s", "sample": args.per_op_traj,
                               "batched_speedup": round(value / po, 1),

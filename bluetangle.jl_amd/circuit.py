@@ -230,6 +230,16 @@ class Circuit:
         nn = min(cap, nb.value // norm_every) if norm_every > 0 else 0
         return d, norms[:nn].tolist()
 
+    def autotune_fused(self, state: StateVector):
+        """Pick the fusion width for this noiseless circuit on states like
+        `state` (max_k 3, 4, 5 or the register-tile executor, timed on a
+        scratch state); run_fused(max_k=0) then uses it.  Returns (best k,
+        device ms of k = 3, 4, 5 and the register-tile executor)."""
+        k = ctypes.c_int()
+        ms = np.zeros(4, dtype=np.float64)
+        L.check(L.lib().bt_autotune_fused(state.handle, self.handle, ctypes.byref(k), L.dptr(ms)))
+        return k.value, ms.tolist()
+
     def capture(self, state: StateVector, max_k=5):
         """The fused form of this (noiseless, gate-only) circuit captured as a
         HIP graph bound to `state`; call .launch() to run it again."""

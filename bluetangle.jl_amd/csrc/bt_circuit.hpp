@@ -368,4 +368,5 @@ struct bt_circuit_s {
   bti::ChanDesc* d_chan = nullptr;
   double* d_chanw = nullptr;
   std::mutex mu;
+  int fused_k = -1;  // bt_autotune_fused's choice (0: register-tile executor), -1: not tuned
 };

@@ -205,14 +205,15 @@ static int g_tpb_override = 0;
 
 int tiles_per_block() { return g_tpb_override ? g_tpb_override : 8; }
 
-// Tiles per workgroup a rounds launch uses: the setting, capped so that the
-// largest item still spreads over at least 32 workgroups and the launch has
-// about eight workgroups per CU (a workgroup walks its tiles in turn, so a
-// launch with little total work would otherwise run long on few workgroups).
-int effective_tpb(int setting, uint64_t max_tiles, uint64_t total_tiles) {
-  const int cus = ctx().cus > 0 ? ctx().cus : 256;
-  const uint64_t by_total = total_tiles / (uint64_t)(8 * cus);
-  return (int)std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)setting, max_tiles / 32, by_total}));
+// Tiles per workgroup of a rounds item of ntiles tiles: the setting, capped
+// so that the item still spreads over at least ntiles / cap_div workgroups
+// (a workgroup walks its tiles in turn).  It depends on the item alone, so
+// an item's reductions, summed per workgroup and then over its workgroups in
+// a fixed order, are the same whatever else a launch holds.  The batch
+// executor uses cap_div = 32, the server 256 (a lone request still fills the
+// chip: latency).
+int item_tpb(int setting, uint64_t ntiles, int cap_div) {
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)setting, ntiles / (uint64_t)cap_div));
 }
 
 int default_tile_bits(int prec) {
@@ -624,11 +625,10 @@ int bt_get_tuning(int prec, int* tile_bits, int* coalesce, int* tiles_per_blk) {
 // of the fastest (a run's workgroup count sets what the per-workgroup costs
 // weigh: pass tables staged into LDS, one partial sum each; a sample smaller
 // than the run underweights them).  Stage 2 times each distinct effective
-// value once (effective_tpb: candidates the caps turn into the same launch
-// are one candidate) and installs and reports the effective value; a
-// candidate that the sample's total work would cap below what a full run
-// uses is not timed unless no candidate is left.  The first failing run
-// ends the search with its error and the previous tuning restored.
+// value once (item_tpb: candidates the cap turns into the same launch are
+// one candidate) and installs and reports the effective value.  The first
+// failing run ends the search with its error and the previous tuning
+// restored.
 int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_tuning* best) {
   if (!c || !best) return fail(BT_ERR_ARG, "null circuit or result");
   if (prec != BT_C64 && prec != BT_C128) return fail(BT_ERR_ARG, "bad precision");
@@ -670,18 +670,14 @@ int bt_autotune_batch(bt_circuit c, int prec, int64_t sample, uint64_t seed, bt_
       }
     }
   if (rc == BT_OK && best_ms >= 0) {
-    const uint64_t max_tiles = 1ull << (nb - std::min(nb, b.tile_bits));
-    const uint64_t total = (uint64_t)sample2 * max_tiles;
+    const uint64_t ntiles = 1ull << (nb - std::min(nb, b.tile_bits));
     int cand[4], nc = 0;
-    for (int pass = 0; pass < 2 && nc == 0; pass++)
-      for (int want : {4, 8, 16, 32}) {
-        const int eff = effective_tpb(want, max_tiles, total);
-        const int uncapped = effective_tpb(want, max_tiles, ~0ull >> 1);
-        if (pass == 0 && eff != uncapped) continue;  // the sample's size caps it below a full run's
-        bool dup = false;
-        for (int i = 0; i < nc; i++) dup = dup || cand[i] == eff;
-        if (!dup) cand[nc++] = eff;
-      }
+    for (int want : {4, 8, 16, 32}) {
+      const int eff = item_tpb(want, ntiles, 32);
+      bool dup = false;
+      for (int i = 0; i < nc; i++) dup = dup || cand[i] == eff;
+      if (!dup) cand[nc++] = eff;
+    }
     double t2[4];
     double best2 = -1;
     for (int i = 0; i < nc && rc == BT_OK; i++) {

@@ -234,6 +234,16 @@ extern "C" int bt_run_fused(bt_state s, bt_circuit c, int max_k, int norm_every,
   if (!s || !c) return fail(BT_ERR_ARG, "null state or circuit");
   if (s->kind != BT_SV) return fail(BT_ERR_ARG, "bt_run_fused: state vectors only");
   if (s->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
+  if (max_k == 0) {  // the autotuner's choice (5 when not tuned)
+    max_k = c->fused_k < 0 ? 5 : c->fused_k;
+    if (max_k == 0) {  // the register-tile executor: one pass per tile plan, no norm checks in between
+      int64_t nout = 0;
+      rc = bt_run_circuit(s, c, 0, 0, nullptr, 0, nullptr, 0, &nout, stats);
+      if (rc == BT_OK && nblocks) *nblocks = stats ? stats->passes : 0;
+      if (rc == BT_OK && norms && norm_every > 0 && nnorms > 0) rc = bt_norm2(s, norms);
+      return rc;
+    }
+  }
   const bool mfma = s->prec == BT_C64 && max_k == 5 && s->n >= kF5T;
   std::vector<btf::Block> blocks;
   std::string msg;
@@ -384,5 +394,48 @@ extern "C" int bt_graph_destroy(bt_graph g) {
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->d_al) (void)hipFree(g->d_al);
   delete g;
+  return BT_OK;
+}
+
+extern "C" int bt_autotune_fused(bt_state s, bt_circuit c, int* best_k, double* ms) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (!s || !c || !best_k) return fail(BT_ERR_ARG, "null argument");
+  if (s->kind != BT_SV) return fail(BT_ERR_ARG, "bt_autotune_fused: state vectors only");
+  if (s->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
+  bt_state scratch = nullptr;
+  rc = bt_state_create(s->n, BT_SV, s->prec, &scratch);
+  if (rc) return rc;
+  double t[4] = {-1, -1, -1, -1};
+  int best = -1;
+  double best_ms = 0;
+  for (int i = 0; i < 4 && rc == BT_OK; i++) {
+    const int k = i < 3 ? 3 + i : 0;
+    if (k == 5 && !(s->prec == BT_C64 && s->n >= kF5T)) continue;  // MFMA blocks: complex64, N >= 12
+    bt_run_stats st;
+    memset(&st, 0, sizeof st);
+    rc = bt_state_set_basis(scratch, 0);
+    if (rc) break;
+    if (k) {
+      int64_t nb = 0;
+      rc = bt_run_fused(scratch, c, k, 0, nullptr, 0, &nb, &st);
+    } else {
+      int64_t nout = 0;
+      rc = bt_run_circuit(scratch, c, 0, 0, nullptr, 0, nullptr, 0, &nout, &st);
+    }
+    if (rc) break;
+    t[i] = st.kernel_ms;
+    if (best < 0 || t[i] < best_ms) {
+      best = k;
+      best_ms = t[i];
+    }
+  }
+  (void)bt_state_destroy(scratch);
+  if (rc) return rc;
+  if (best < 0) return fail(BT_ERR_STATE, "bt_autotune_fused: no candidate ran");
+  c->fused_k = best;
+  *best_k = best;
+  if (ms)
+    for (int i = 0; i < 4; i++) ms[i] = t[i];
   return BT_OK;
 }

@@ -35,7 +35,7 @@ namespace bt {
 int tile_bits_for(int prec, int nbits);
 int coalesce_bits(int prec);
 int tiles_per_block();
-int effective_tpb(int setting, uint64_t max_tiles, uint64_t total_tiles);
+int item_tpb(int setting, uint64_t ntiles, int cap_div);
 
 // ---------------------------------------------------------------------------
 // device side
@@ -43,30 +43,27 @@ int effective_tpb(int setting, uint64_t max_tiles, uint64_t total_tiles);
 struct RoundItem {
   void* state;
   uint64_t table;     // byte offset of the job's table in the round's blob
-  uint64_t part0;     // first per-tile reduction slot of this item
   uint32_t ntiles;    // tiles of the pass
   uint32_t block0;    // first workgroup of this item
   uint32_t nblocks;
+  uint32_t tpb;       // tiles per workgroup of this item
   int32_t fresh;      // state is |0..0> and not yet written
+  int32_t pad;
 };
-
-// reduction slots per tile: one per wavefront of the workgroup
-template <typename R>
-constexpr int kSlotsPerTile = TileCfg<R>::NT / 64;
 
 // table layout: PassGeom, then int32 nmats (padded to 16 B), then nmats matrix entries
 constexpr size_t kGeomBytes = (sizeof(PassGeom) + 15) / 16 * 16;
 
 
-// One workgroup per `tpb` consecutive tiles of one item.  nb_uniform != 0:
-// every item has that many workgroups (the block -> item map is a division),
-// else a binary search over the items' first blocks.  Reductions go to one
-// slot per tile and wavefront (part[j * nslots + slot]), so an item's sums do
-// not depend on tpb or on the other items of the launch.
+// One workgroup per `tpb` consecutive tiles of one item (tpb per item, from
+// the item's own size: an item's reductions, summed per workgroup and then
+// over its workgroups in a fixed order, do not depend on what else the
+// launch holds).  nb_uniform != 0: every item has that many workgroups (the
+// block -> item map is a division), else a binary search over the items'
+// first blocks.
 template <typename R>
 __global__ void __launch_bounds__(TileCfg<R>::NT, TileCfg<R>::WAVES) k_round(const RoundItem* items, int nitems, uint32_t nb_uniform,
-                                                          const unsigned char* blob, int tpb, double* part,
-                                                          uint64_t nslots) {
+                                                          const unsigned char* blob, double* part) {
   constexpr int NT = TileCfg<R>::NT;
   extern __shared__ __align__(16) unsigned char smem[];
   C<R>* tile = (C<R>*)smem;
@@ -102,19 +99,18 @@ __global__ void __launch_bounds__(TileCfg<R>::NT, TileCfg<R>::WAVES) k_round(con
   if (threadIdx.x < kAcc) sums[threadIdx.x] = 0.0;
   __syncthreads();
   const uint32_t local = blockIdx.x - it.block0;
-  const uint64_t t0 = (uint64_t)local * tpb;
-  const uint64_t tend = std::min<uint64_t>((uint64_t)it.ntiles, t0 + tpb);
-  tile_pass<R, uint32_t>((C<R>*)it.state, geom, mats, tile, t0, 1, tend, red, sums, it.fresh, part, nslots,
-                         it.part0);
+  const uint64_t t0 = (uint64_t)local * it.tpb;
+  const uint64_t tend = std::min<uint64_t>((uint64_t)it.ntiles, t0 + it.tpb);
+  tile_pass<R, uint32_t>((C<R>*)it.state, geom, mats, tile, t0, 1, tend, red, sums, it.fresh);
+  if (threadIdx.x < kAcc) part[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = sums[threadIdx.x];
 }
 
-// out[item, j]: fixed-order sum of the item's per-tile slots (grid: items x
-// nacc, the accumulators the round's reductions use)
+// out[item, j]: fixed-order sum of the item's partials (grid: items x nacc,
+// the accumulators the round's reductions use)
 __global__ void __launch_bounds__(256) k_round_finalize(const RoundItem* items, const double* part, uint64_t stride,
-                                                        int slots_per_tile, double* out) {
+                                                        double* out) {
   const RoundItem it = items[blockIdx.x];
-  const double t = sum_blocks(part + (uint64_t)blockIdx.y * stride + it.part0,
-                              (uint32_t)((uint64_t)it.ntiles * (uint64_t)slots_per_tile));
+  const double t = sum_blocks(part + (uint64_t)blockIdx.y * stride + it.block0, it.nblocks);
   if (threadIdx.x == 0) out[(uint64_t)blockIdx.x * gridDim.y + blockIdx.y] = t;
 }
 
@@ -370,18 +366,9 @@ class RoundEngine {
     // launch has workgroups for several waves of the chip's slots — a
     // workgroup walks its tiles in turn, so in a launch with little total
     // work (a serving round) many tiles per workgroup only lengthen it
-    uint32_t max_tiles = 0;
-    uint64_t total_tiles = 0;
-    for (Job* j : act) {
-      const uint64_t nt = 1ull << (j->nbits - j->plan.ntile);
-      max_tiles = std::max<uint32_t>(max_tiles, (uint32_t)nt);
-      total_tiles += nt;
-    }
-    const int tpb = effective_tpb(tiles_per_block(), max_tiles, total_tiles);
+    const int setting = tiles_per_block();
     size_t blob = 0;
     uint32_t blocks = 0;
-    uint64_t nslots = 0;
-    const int spt = prec_ == BT_C64 ? kSlotsPerTile<float> : kSlotsPerTile<double>;
     std::vector<RoundItem> items(act.size());
     size_t maxT = 0;
     uint32_t nb_uniform = 0;
@@ -393,10 +380,9 @@ class RoundEngine {
       it.state = j->state;
       it.table = blob;
       it.ntiles = (uint32_t)(1ull << (j->nbits - T));
-      it.part0 = nslots;
-      nslots += (uint64_t)it.ntiles * (uint64_t)spt;
+      it.tpb = (uint32_t)item_tpb(setting, it.ntiles, cap_div_);
       it.block0 = blocks;
-      it.nblocks = (it.ntiles + tpb - 1) / tpb;
+      it.nblocks = (it.ntiles + it.tpb - 1) / it.tpb;
       it.fresh = j->fresh;
       if (i == 0) nb_uniform = it.nblocks;
       else if (nb_uniform != it.nblocks) nb_uniform = 0;
@@ -406,7 +392,7 @@ class RoundEngine {
     nacc_ = kAccSmall;
     for (Job* j : act)
       if (j->plan.red == bti::RED_RHO && j->plan.red_k == 3) nacc_ = kAcc;
-    int rc = reserve(blob, act.size(), nslots * (uint64_t)nacc_);
+    int rc = reserve(blob, act.size(), (uint64_t)blocks * (uint64_t)kAcc);
     if (rc) return rc;
     pool_.parallel_for((int)act.size(), [&](int i) {
       memcpy((unsigned char*)h_blob_ + items[i].table, act[i]->table.data(), act[i]->table.size());
@@ -420,16 +406,16 @@ class RoundEngine {
     if (prec_ == BT_C64) {
       (void)hipFuncSetAttribute((const void*)k_round<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL((k_round<float>), dim3(blocks), dim3(TileCfg<float>::NT), lds, stream_, d_items_,
-                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, tpb, d_part_, nslots);
+                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, d_part_);
     } else {
       (void)hipFuncSetAttribute((const void*)k_round<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL((k_round<double>), dim3(blocks), dim3(TileCfg<double>::NT), lds, stream_, d_items_,
-                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, tpb, d_part_, nslots);
+                         (int)items.size(), nb_uniform, (const unsigned char*)d_blob_, d_part_);
     }
     BT_HIP(hipGetLastError());
     BT_HIP(hipEventRecord(ev_b_, stream_));
     hipLaunchKernelGGL(k_round_finalize, dim3((unsigned)items.size(), (unsigned)nacc_), dim3(256), 0, stream_,
-                       d_items_, d_part_, nslots, spt, d_out_);
+                       d_items_, d_part_, (uint64_t)blocks, d_out_);
     BT_HIP(hipGetLastError());
     BT_HIP(hipMemcpyAsync(h_out_, d_out_, sizeof(double) * nacc_ * items.size(), hipMemcpyDeviceToHost, stream_));
     BT_HIP(hipEventRecord(ev_done_, stream_));
@@ -464,6 +450,8 @@ class RoundEngine {
     j->bytes_moved = j->ip->bytes_units * (double)(1ull << j->nbits) * (double)elem;
   }
 
+  // items get at least ntiles / cap_div workgroups (see item_tpb)
+  void set_cap_div(int d) { cap_div_ = d; }
   int64_t rounds() const { return rounds_; }
   double pass_ms() const { return pass_ms_; }
   void reset_counters() {
@@ -474,7 +462,7 @@ class RoundEngine {
 
  private:
   // pass tables (bytes), items, reduction slots (doubles)
-  int reserve(size_t blob, size_t nitems, uint64_t part_doubles) {
+  int reserve(size_t blob, size_t nitems, uint64_t part_doubles) {  // part: nacc rows x workgroups
     if (blob > cap_blob_) {
       if (h_blob_) (void)hipHostFree(h_blob_);
       if (d_blob_) (void)hipFree(d_blob_);
@@ -521,6 +509,7 @@ class RoundEngine {
   double* h_out_ = nullptr;
   int64_t rounds_ = 0;
   int nacc_ = kAccSmall;
+  int cap_div_ = 32;
   uint64_t launched_blocks_ = 0;
   hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
   double pass_ms_ = 0;
@@ -951,6 +940,8 @@ static void server_loop(bt_server_s* S) {
     if (!S->eng[e][0]) {
       std::unique_ptr<RoundEngine> a(new RoundEngine(prec)), b(new RoundEngine(prec));
       if (a->init() || b->init(a->stream())) return nullptr;
+      a->set_cap_div(256);  // latency: a lone request still spreads over the chip
+      b->set_cap_div(256);
       S->eng[e][0] = std::move(a);
       S->eng[e][1] = std::move(b);
     }

@@ -798,61 +798,30 @@ __device__ __forceinline__ double wave_sum(double x) {
 // The quad's four partials of a 3-bit reduced-rho entry, summed in a fixed
 // order ((s0 + s1) + (s2 + s3)), added by the quad's first lane to its
 // wavefront's packed-rho slots (wrho: LDS; each entry has one writer).
-// Where a tile pass puts its FP64 reductions.  Without tpart, each
-// wavefront adds its tiles' values to its accumulators in LDS (wred) and the
-// waves are summed at the end: one partial per workgroup.  With tpart, each
-// tile's values go to slots of their own, tpart[j * tstride + slot] (slot =
-// the wavefront's slot for the current tile), to be summed later in a fixed
-// order, so the result does not depend on how tiles are grouped into
-// workgroups.
-struct RedSink {
-  double* wred;
-  double* tpart;
-  uint64_t tstride;
-  uint64_t slot;
-};
-
-__device__ __forceinline__ void sink_put(const RedSink& k, int j, double t) {
-  if (k.tpart) k.tpart[(uint64_t)j * k.tstride + k.slot] = t;
-  else k.wred[j] += t;
-}
-
 template <typename R>
-__device__ __forceinline__ void rho3_accumulate_t(const C<R>* lds, const PassGeom& g, const RedSink& k) {
+__device__ __forceinline__ void rho3_accumulate_t(const C<R>* lds, const PassGeom& g, double* wrho) {
   double xr, xi;
   const int e = rho3_partial<R>(lds, g, threadIdx.x, &xr, &xi);
   xr += dpp_f64<0xb1>(xr);
   xr += dpp_f64<0x4e>(xr);
   xi += dpp_f64<0xb1>(xi);
   xi += dpp_f64<0x4e>(xi);
-  if (!k.tpart) {
-    if (e >= 0 && (threadIdx.x & 3) == 0) {
-      const int sl = rho3_slot(e);
-      k.wred[1 + sl] += xr;
-      if (e >= 8) k.wred[2 + sl] += xi;
-    }
-    return;
+  if (e >= 0 && (threadIdx.x & 3) == 0) {
+    const int sl = rho3_slot(e);
+    wrho[sl] += xr;
+    if (e >= 8) wrho[sl + 1] += xi;
   }
-  // per-tile slots: lane i of the wavefront writes packed slot i (row 1 + i),
-  // its entry's quad sum when this wavefront owns the entry, else zero
-  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-  const int ei = lane < 8 ? lane : 8 + (lane - 8) / 2;
-  const bool imag = lane >= 8 && ((lane - 8) & 1);
-  const bool own = ei < 36 && ei / 16 == wave;
-  const int src = own ? 4 * (ei - 16 * wave) : lane;
-  const double vr = __shfl(xr, src, 64), vi = __shfl(xi, src, 64);
-  k.tpart[(uint64_t)(1 + lane) * k.tstride + k.slot] = own ? (imag ? vi : vr) : 0.0;
 }
 
-// Add M per-thread values into this wavefront's accumulator slots j0 ..
-// j0 + M - 1: one writer lane, tiles in order, no barrier.
+// Add M per-thread values into this wavefront's accumulator slots
+// (wred[0..M), LDS): one writer lane, tiles in order, no barrier.
 template <int M>
-__device__ __forceinline__ void wave_accumulate(const double* v, const RedSink& k, int j0 = 0) {
+__device__ __forceinline__ void wave_accumulate(const double* v, double* wred) {
   const bool lead = (threadIdx.x & 63) == 0;
 #pragma unroll
   for (int j = 0; j < M; j++) {
     const double t = wave_sum(v[j]);
-    if (lead) sink_put(k, j0 + j, t);
+    if (lead) wred[j] += t;
   }
 }
 #endif
@@ -1188,24 +1157,22 @@ __device__ __forceinline__ double sum_blocks(const double* __restrict__ p, uint3
   return sh[0];
 }
 
-// One pass over tiles t = t0, t0 + tstep, ... < tend.  Reductions ([0]
-// norm^2, then the histogram or the packed reduced rho) are either added to
-// sums[0..kAcc) (LDS, zeroed by the caller; each wavefront accumulates its
-// tiles in red[wave * kAcc + j], LDS, NT/64 * kAcc doubles, and the waves
-// are summed in order at the end) or, with tpart, stored per tile and
-// wavefront: tpart[j * tstride + part0 + (NT/64) * t + wave].  lds holds 2^T
-// elements.  fresh != 0: the state is |0...0> and is not read (its first
-// pass).  The host emulation of this loop (tests/emu) follows it step for step.
+// One pass over tiles t = t0, t0 + tstep, ... < tend.  Reductions are added
+// to sums[0..kAcc) (LDS, zeroed by the caller): [0] norm^2, [1..16]
+// histogram, [17..32] packed rho; each wavefront accumulates its tiles in
+// red[wave * kAcc + j] (LDS, NT/64 * kAcc doubles) and the waves are summed
+// in order at the end.  lds holds 2^T elements.
+// fresh != 0: the state is |0...0> and is not read (its first pass).
+// The host emulation of this loop (tests/emu) follows it step for step.
 template <typename R, typename O>
 __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGeom& g, const C<R>* mats, C<R>* lds,
                                           uint64_t t0, uint64_t tstep, uint64_t tend, double* red, double* sums,
-                                          int fresh = 0, double* tpart = nullptr, uint64_t tstride = 0,
-                                          uint64_t part0 = 0) {
+                                          int fresh = 0) {
   constexpr int NT = TileCfg<R>::NT;
   constexpr int NR = 1 << TileCfg<R>::RB;
   const int T = g.T;
   const R scale = (R)g.scale;
-  RedSink sink{red + (threadIdx.x >> 6) * kAcc, tpart, tstride, 0};
+  double* wred = red + (threadIdx.x >> 6) * kAcc;
   for (int i = threadIdx.x; i < (NT / 64) * kAcc; i += NT) red[i] = 0.0;
   __syncthreads();
   ThreadIO<O> io;
@@ -1217,7 +1184,6 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
   uint64_t base = uniform_u64(tile_base(g, t0));
   if (!fresh && t0 < tend) fetch_regs<R, O>(nx, state, io, base);
   for (uint64_t t = t0; t < tend; t += tstep) {
-    sink.slot = part0 + (uint64_t)(NT / 64) * t + (threadIdx.x >> 6);
     opaque_io(io);
     const uint64_t nbase = ((base | ~M) + S) & M;
     take_regs<R, O>(v, nx, io, base, scale, fresh);
@@ -1267,14 +1233,14 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
         if (uflag(g.red_k == 3)) {
           regs_to_lds<R>(v, lds, g.lay[ci], io.tid, io.active, io.nvalid_mask);
           __syncthreads();
-          rho3_accumulate_t<R>(lds, g, sink);
+          rho3_accumulate_t<R>(lds, g, wred + 1);
           __syncthreads();
         } else {
           double racc[16];
 #pragma unroll
           for (int j = 0; j < 16; j++) racc[j] = 0.0;
           if (io.active) rho_mask<R>(rho_reg_mask(g), v, io.nvalid_mask, racc);
-          wave_accumulate<16>(racc, sink, 1);
+          wave_accumulate<16>(racc, wred + 1);
         }
       }
     }
@@ -1284,13 +1250,13 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
     store_regs<R, O>(v, state, g, io, base, acc);
     if (g.red == bti::RED_HIST) {
       switch (g.red_k) {  // norm + 2^red_k bins
-        case 1: wave_accumulate<3>(acc, sink); break;
-        case 2: wave_accumulate<5>(acc, sink); break;
-        case 3: wave_accumulate<9>(acc, sink); break;
-        default: wave_accumulate<17>(acc, sink); break;
+        case 1: wave_accumulate<3>(acc, wred); break;
+        case 2: wave_accumulate<5>(acc, wred); break;
+        case 3: wave_accumulate<9>(acc, wred); break;
+        default: wave_accumulate<17>(acc, wred); break;
       }
     } else if (g.need_norm) {
-      wave_accumulate<1>(acc, sink);
+      wave_accumulate<1>(acc, wred);
     }
     base = nbase;
   }

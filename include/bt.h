@@ -212,14 +212,14 @@ int bt_run_batch(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t n
 int bt_run_batch_sample(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
                         int64_t outcap, int64_t* nouts, uint64_t* samples, int64_t* sample_boundary,
                         bt_run_stats* stats);
-/* The same contract in ONE persistent launch: each workgroup owns a
- * trajectory, runs its interpreter on the device (lane 0) and takes the next
- * trajectory from an atomic counter. */
 /* bt_run_batch keeps its worker threads, stream and the slab of resident
  * states between calls (a repeated batch pays no set-up); this frees the
  * cached slab.  Other allocations of the library that run out of HBM free it
  * too.  No counterpart in the reference (which allocates nothing on a device). */
 int bt_release_cache(void);
+/* The same contract in ONE persistent launch: each workgroup owns a
+ * trajectory, runs its interpreter on the device (lane 0) and takes the next
+ * trajectory from an atomic counter. */
 int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes,
                             int64_t outcap, int64_t* nouts, double* final_states, bt_run_stats* stats);
 
@@ -235,6 +235,16 @@ int bt_apply_fused5(bt_state s, const double* mat, const int* bits);
  * *nblocks: blocks applied. */
 int bt_run_fused(bt_state s, bt_circuit c, int max_k, int norm_every, double* norms, int64_t nnorms, int64_t* nblocks,
                  bt_run_stats* stats);
+/* Fusion-width autotuner for a noiseless gate-only circuit on states like s
+ * (its qubit count and precision): times bt_run_fused at max_k = 3, 4 and 5
+ * and the register-tile executor (bt_run_circuit) on a scratch state of the
+ * same size, and records the fastest on the circuit; bt_run_fused with
+ * max_k = 0 then runs it.  *best_k: 3, 4 or 5, or 0 for the register-tile
+ * executor; ms (nullable): the four device times in that order (k = 3, 4, 5,
+ * tile executor; < 0 where a candidate does not apply).  The results do not
+ * depend on the choice beyond rounding (every executor is checked against the
+ * oracle).  No counterpart in the reference. */
+int bt_autotune_fused(bt_state s, bt_circuit c, int* best_k, double* ms);
 
 /* A fused noiseless circuit captured as a HIP graph on state s: every pass
  * of bt_run_fused recorded once and replayed by bt_graph_launch (one launch,

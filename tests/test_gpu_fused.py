@@ -122,3 +122,26 @@ def test_captured_graph_replays(bt, prec, max_k):
         assert abs(n - eager.norm2()) < 10 * tol
         assert np.abs(g_state.to_numpy() - eager.to_numpy()).max() < 10 * tol, rep
     g.close()
+
+
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_fusion_width_autotuner(bt, prec):
+    """bt_autotune_fused times max_k = 3, 4, 5 and the register-tile executor
+    on a scratch state and records the fastest on the circuit; run_fused with
+    max_k = 0 runs it, with the oracle's result (the caller's state is not
+    touched by the tuning)."""
+    from bluetangle_jl_amd.circuit import Circuit
+    N = 14
+    spec = brickwork_spec(N, 6, 31)
+    c = Circuit(to_bt(spec, bt), N)
+    st = bt.zero_state(N, prec)
+    st.set_basis(5)
+    k, ms = c.autotune_fused(st)
+    assert k in (0, 3, 4, 5)
+    assert ms[0] > 0 and ms[1] > 0 and ms[3] > 0
+    assert (ms[2] > 0) == (prec == "c64")  # MFMA blocks: complex64 only
+    assert np.abs(st.to_numpy() - np.eye(1, 2 ** N, 5)[0]).max() == 0
+    st.set_basis(0)
+    c.run_fused(st, max_k=0)
+    r, _, _, _, _ = O.run_sv(N, to_ref(spec), None)
+    assert np.abs(st.to_numpy() - r).max() < TOL[prec]

@@ -195,11 +195,11 @@ def test_arrivals_during_execution(bt):
 
 
 def test_mix_equals_alone_whatever_the_grouping(bt):
-    """A request's reductions are summed per tile in a fixed order, so its
-    result does not depend on how many tiles a workgroup takes, which the
-    engine sets from the whole launch: an N=18 request alone (one tile per
-    workgroup) and beside an N=26 request (several per workgroup) gives the
-    same outcomes and probabilities bit for bit."""
+    """A request's tiles per workgroup (and with it the grouping of its FP64
+    reductions) follows from its own size and the tuning alone, not from the
+    other requests of a launch: an N=18 request alone and beside an N=26
+    request (tiles per workgroup set to 32) gives the same outcomes and
+    probabilities bit for bit."""
     from bluetangle_jl_amd import _lib
     from bluetangle_jl_amd.circuit import Circuit, Server
     from bluetangle_jl_amd import workloads as W
