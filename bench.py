@@ -259,9 +259,27 @@ def small_configs_leg(bt, args):
     dt = (time.perf_counter() - t0) / reps
     tr, pu = r64.trace_purity()
     err = float(np.abs(r64.to_numpy() - r128.to_numpy()).max())
+    # the same noisy density-matrix circuit captured once in a HIP graph on a
+    # resident rho, replayed per circuit (reset + replay + FP64 trace/purity)
+    rg = bt.zero_rho(c2["N"], "c64")
+    graph = cd.capture_dm(rg)
+    rg.set_basis(0)
+    graph.launch()
+    rg.trace_purity()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rg.set_basis(0)
+        graph.launch()
+        trg, pug = rg.trace_purity()
+    dtg = (time.perf_counter() - t0) / reps
+    gdiff = float(np.abs(rg.to_numpy() - r64.to_numpy()).max())
     out["c2"] = dict(N=c2["N"], dtype="complex64 (FP64 trace/purity)", layers=16, ms_per_circuit=round(dt * 1e3, 3),
                      max_entry_err_vs_fp64=err, trace=tr, purity=pu,
-                     note="bt_run_circuit_dm: gates as E rho E^dagger, channels as one superoperator pass each")
+                     note="bt_run_circuit_dm: gates as E rho E^dagger, channels as one superoperator pass each",
+                     graph_ms_per_circuit=round(dtg * 1e3, 3), graph_passes=graph.passes, graph_max_entry_diff=gdiff,
+                     graph_note="the density-matrix passes captured once in a HIP graph (bt_graph_create_dm), "
+                                "replayed per circuit (reset + replay + FP64 trace and purity)")
+    del graph, rg
     return out
 
 

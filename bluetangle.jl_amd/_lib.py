@@ -25,7 +25,7 @@ SYMBOLS = [
     "bt_last_error", "bt_version", "bt_init", "bt_uniform",
     "bt_state_create", "bt_state_destroy", "bt_state_info", "bt_state_set_basis", "bt_state_copy",
     "bt_state_upload", "bt_state_download", "bt_state_device_ptr", "bt_state_sync", "bt_state_bytes",
-    "bt_apply_full", "bt_apply_fused5", "bt_run_fused", "bt_autotune_fused", "bt_inner", "bt_expect", "bt_sample", "bt_sample_noalloc", "bt_run_batch_sample", "bt_graph_create_fused", "bt_graph_launch",
+    "bt_apply_full", "bt_apply_fused5", "bt_run_fused", "bt_autotune_fused", "bt_inner", "bt_expect", "bt_sample", "bt_sample_noalloc", "bt_run_batch_sample", "bt_graph_create_fused", "bt_graph_create_dm", "bt_graph_launch",
     "bt_graph_passes", "bt_graph_destroy",
     "bt_apply_matrix", "bt_reduced_rho", "bt_apply_kraus_traj", "bt_apply_kraus_dm", "bt_measure", "bt_reset",
     "bt_probabilities", "bt_norm2", "bt_trace_purity", "bt_gather",
@@ -154,6 +154,7 @@ def lib():
             "bt_server_fetch": (ctypes.c_int, [vp, ctypes.c_int64, ip, ctypes.c_int64, i64p, dp, ctypes.c_uint64]),
             "bt_server_stats": (ctypes.c_int, [vp, ip, ip, u64p]),
             "bt_autotune_fused": (ctypes.c_int, [vp, vp, ip, dp]),
+            "bt_graph_create_dm": (ctypes.c_int, [vp, vp, ctypes.POINTER(vp)]),
             "bt_server_reserve": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]),
             "bt_server_destroy": (ctypes.c_int, [vp]),
             "bt_release_cache": (ctypes.c_int, []),

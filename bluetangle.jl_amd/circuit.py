@@ -240,6 +240,12 @@ class Circuit:
         L.check(L.lib().bt_autotune_fused(state.handle, self.handle, ctypes.byref(k), L.dptr(ms)))
         return k.value, ms.tolist()
 
+    def capture_dm(self, rho: DensityMatrix):
+        """to_rho of this circuit captured as a HIP graph on `rho`
+        (bt_graph_create_dm); .launch() replays the whole circuit (reset rho
+        with set_basis(0) first)."""
+        return FusedGraph(self, rho, dm=True)
+
     def capture(self, state: StateVector, max_k=5):
         """The fused form of this (noiseless, gate-only) circuit captured as a
         HIP graph bound to `state`; call .launch() to run it again."""
@@ -398,12 +404,16 @@ def run(ops_or_circuit, number_of_experiment: int = 1, N=None, noise=False, seed
 
 
 class FusedGraph:
-    """A captured fused circuit (bt_graph_*): launch() enqueues every pass at
-    once on the state's stream; state.sync() waits."""
+    """A captured circuit (bt_graph_*): a fused noiseless circuit on a state
+    vector, or a density-matrix circuit (dm=True); launch() enqueues every
+    pass at once on the state's stream; state.sync() waits."""
 
-    def __init__(self, circuit: Circuit, state: StateVector, max_k=5):
+    def __init__(self, circuit: Circuit, state, max_k=5, dm=False):
         h = ctypes.c_void_p()
-        L.check(L.lib().bt_graph_create_fused(state.handle, circuit.handle, max_k, ctypes.byref(h)))
+        if dm:
+            L.check(L.lib().bt_graph_create_dm(state.handle, circuit.handle, ctypes.byref(h)))
+        else:
+            L.check(L.lib().bt_graph_create_fused(state.handle, circuit.handle, max_k, ctypes.byref(h)))
         self.handle = h
         self.state = state  # the graph is bound to this state's memory
         self.passes = int(L.lib().bt_graph_passes(h))

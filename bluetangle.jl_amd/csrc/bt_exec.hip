@@ -233,6 +233,14 @@ int tile_bits_for(int prec, int nbits) {
 }
 
 
+static_assert(kGridAccRows == kAcc, "grid pass partial rows");
+
+// Workgroups of a grid pass over ntiles tiles: eight per CU at most.
+static unsigned grid_pass_blocks(uint64_t ntiles) {
+  const int cus = ctx().cus > 0 ? ctx().cus : 256;
+  return (unsigned)std::min<uint64_t>(ntiles, (uint64_t)cus * 8);
+}
+
 // Host driver of the interpreter over grid passes on state s (SV or DM).
 static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, uint64_t traj, const double* uni,
                     int64_t nuni, int32_t* outs, int64_t outcap, int64_t* nout, bt_run_stats* stats, int fold,
@@ -255,8 +263,7 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
   ip->tol_exact = s->prec == BT_C64 ? 1e-6 : 1e-12;
   const size_t tabsz = s->prec == BT_C64 ? sizeof(GridTable<float>) : sizeof(GridTable<double>);
   const uint64_t ntiles = 1ull << (nbits - std::min(T, nbits));
-  const int cus = ctx().cus > 0 ? ctx().cus : 256;
-  const unsigned blocks = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)cus * 8);
+  const unsigned blocks = grid_pass_blocks(ntiles);
   const size_t lds = (size_t)(1u << std::min(T, nbits)) * s->elem();
   constexpr int kRing = 8;
   void* h_ring = nullptr;
@@ -392,6 +399,88 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
   return rc;
 }
 
+// Pass tables of a circuit whose pass plan does not depend on reductions
+// (a density matrix: no sampling, no renormalisation, fold = 0), planned on
+// the host once: tables gets npass GridTable entries of tabsz bytes each,
+// narrow[i] whether pass i uses 32-bit in-tile offsets.
+int plan_static_passes(bt_state_s* s, const btc::CircuitHost& ch, int T, std::vector<unsigned char>* tables,
+                       std::vector<uint8_t>* narrow, size_t* tabsz) {
+  const int nbits = s->nbits;
+  std::vector<QOp> slots(bti::kQMax);
+  std::unique_ptr<bti::Interp> ip(new bti::Interp());
+  bti::Program prog{ch.ops.data(), ch.ntop, ch.pool.data(), ch.chan.data(), ch.chanw.data(), nbits};
+  ip->init(prog, slots.data(), T, coalesce_bits(s->prec));
+  ip->fold = 0;
+  *tabsz = s->prec == BT_C64 ? sizeof(GridTable<float>) : sizeof(GridTable<double>);
+  tables->clear();
+  narrow->clear();
+  Plan plan;
+  PassResult res;
+  memset(&res, 0, sizeof res);
+  const int64_t max_passes = 64 + 8 * (int64_t)ch.ops.size();
+  int64_t npass = 0;
+  while (ip->step(&plan, &res)) {
+    if (plan.red != bti::RED_NONE) return fail(BT_ERR_UNSUPPORTED, "plan_static_passes: a pass needs a reduction");
+    if (++npass > max_passes) return fail(BT_ERR_STATE, "plan_static_passes: pass limit exceeded");
+    tables->resize(tables->size() + *tabsz);
+    unsigned char* t = tables->data() + tables->size() - *tabsz;
+    memset(t, 0, *tabsz);
+    const int grc = s->prec == BT_C64
+                        ? build_geom<float>(plan, slots.data(), nbits, &((GridTable<float>*)t)->g, ((GridTable<float>*)t)->mats)
+                        : build_geom<double>(plan, slots.data(), nbits, &((GridTable<double>*)t)->g,
+                                             ((GridTable<double>*)t)->mats);
+    if (grc) return fail(grc, "plan_static_passes: pass geometry");
+    narrow->push_back(plan.ntile == 0 || plan.tile[plan.ntile - 1] < 32 ? 1 : 0);
+    memset(&res, 0, sizeof res);
+  }
+  if (ip->err) return fail(ip->err, "plan_static_passes: interpreter error (" + std::to_string(ip->err) + ")");
+  return BT_OK;
+}
+
+// Enqueue one grid pass of state s (tile bits T) with its table at dtab;
+// d_part: kAcc x grid doubles of reduction partials (written, unused here).
+int launch_grid_pass(bt_state_s* s, const void* dtab, bool narrow, int T, double* d_part) {
+  const int nbits = s->nbits;
+  const uint64_t ntiles = 1ull << (nbits - std::min(T, nbits));
+  const unsigned blocks = grid_pass_blocks(ntiles);
+  const size_t lds = (size_t)(1u << std::min(T, nbits)) * s->elem();
+  if (s->prec == BT_C64) {
+    if (narrow)
+      hipLaunchKernelGGL((k_grid_pass<float, uint32_t>), dim3(blocks), dim3(TileCfg<float>::NT), lds, s->stream,
+                         (C<float>*)s->d, (const GridTable<float>*)dtab, ntiles, d_part);
+    else
+      hipLaunchKernelGGL((k_grid_pass<float, uint64_t>), dim3(blocks), dim3(TileCfg<float>::NT), lds, s->stream,
+                         (C<float>*)s->d, (const GridTable<float>*)dtab, ntiles, d_part);
+  } else {
+    if (narrow)
+      hipLaunchKernelGGL((k_grid_pass<double, uint32_t>), dim3(blocks), dim3(TileCfg<double>::NT), lds, s->stream,
+                         (C<double>*)s->d, (const GridTable<double>*)dtab, ntiles, d_part);
+    else
+      hipLaunchKernelGGL((k_grid_pass<double, uint64_t>), dim3(blocks), dim3(TileCfg<double>::NT), lds, s->stream,
+                         (C<double>*)s->d, (const GridTable<double>*)dtab, ntiles, d_part);
+  }
+  BT_HIP(hipGetLastError());
+  return BT_OK;
+}
+
+// Allow the grid pass's dynamic LDS tile of 2^T elements.
+void grid_pass_lds_attr(int prec, int T) {
+  const int lds = (int)((size_t)(1u << T) * (prec == BT_C64 ? 8 : 16));
+  if (prec == BT_C64) {
+    (void)hipFuncSetAttribute((const void*)k_grid_pass<float, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void*)k_grid_pass<float, uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  } else {
+    (void)hipFuncSetAttribute((const void*)k_grid_pass<double, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void*)k_grid_pass<double, uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  }
+}
+
+// Tile bits of the density-matrix executor: a superoperator spans up to
+// kMaxQK bits of vec(rho), so a smaller tile setting is raised to hold it.
+int dm_tile_bits(bt_state_s* rho) {
+  return std::max(tile_bits_for(rho->prec, rho->nbits), std::min(rho->nbits, bti::kMaxQK));
+}
+
 }  // namespace bt
 
 using namespace bt;
@@ -450,10 +539,7 @@ int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats) {
   rc = lower_dm(c->h, &dm, &msg);
   if (rc) return fail(rc, msg);
   int64_t nout = 0;
-  // a superoperator spans up to kMaxQK bits of vec(rho): a smaller tile
-  // setting is raised to hold it
-  const int T = std::max(tile_bits_for(rho->prec, rho->nbits), std::min(rho->nbits, bti::kMaxQK));
-  return run_grid(rho, dm, 0, 0, nullptr, 0, nullptr, 0, &nout, stats, 0, T);
+  return run_grid(rho, dm, 0, 0, nullptr, 0, nullptr, 0, &nout, stats, 0, dm_tile_bits(rho));
 }
 
 int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes, int64_t outcap,

@@ -316,6 +316,8 @@ struct bt_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   float* d_al = nullptr;
+  void* d_tab = nullptr;    // density-matrix graphs: the passes' tables
+  double* d_part = nullptr; // and their (unused) reduction partials
   std::vector<btf::Block> blocks;
   int64_t nblocks = 0;
 };
@@ -393,7 +395,63 @@ extern "C" int bt_graph_destroy(bt_graph g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->d_al) (void)hipFree(g->d_al);
+  if (g->d_tab) (void)hipFree(g->d_tab);
+  if (g->d_part) (void)hipFree(g->d_part);
   delete g;
+  return BT_OK;
+}
+
+// to_rho as one HIP graph: the circuit lowered to superoperator passes on
+// vec(rho) (the density-matrix executor's plan, which depends on nothing the
+// passes compute), every pass's table uploaded once and every launch
+// recorded; bt_graph_launch replays the whole circuit on rho.
+extern "C" int bt_graph_create_dm(bt_state rho, bt_circuit c, bt_graph* out) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (!rho || !c || !out) return fail(BT_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (rho->kind != BT_DM) return fail(BT_ERR_ARG, "bt_graph_create_dm needs a density matrix");
+  if (rho->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
+  btc::CircuitHost dm;
+  std::string msg;
+  rc = btc::lower_dm(c->h, &dm, &msg);
+  if (rc) return fail(rc, msg);
+  const int T = dm_tile_bits(rho);
+  std::vector<unsigned char> tables;
+  std::vector<uint8_t> narrow;
+  size_t tabsz = 0;
+  rc = plan_static_passes(rho, dm, T, &tables, &narrow, &tabsz);
+  if (rc) return rc;
+  auto* g = new bt_graph_s();
+  g->s = rho;
+  g->nblocks = (int64_t)narrow.size();
+  auto cleanup = [&](int r) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->d_tab) (void)hipFree(g->d_tab);
+    if (g->d_part) (void)hipFree(g->d_part);
+    delete g;
+    return r;
+  };
+  const int cus = ctx().cus > 0 ? ctx().cus : 256;
+  if (hipMalloc(&g->d_tab, std::max<size_t>(tables.size(), 16)) != hipSuccess ||
+      hipMalloc((void**)&g->d_part, sizeof(double) * kGridAccRows * (size_t)cus * 8) != hipSuccess)
+    return cleanup(fail(BT_ERR_NOMEM, "bt_graph_create_dm: tables"));
+  if (!tables.empty() && hipMemcpy(g->d_tab, tables.data(), tables.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(BT_ERR_HIP, "bt_graph_create_dm: table upload"));
+  grid_pass_lds_attr(rho->prec, std::min(T, rho->nbits));
+  if (hipStreamSynchronize(rho->stream) != hipSuccess) return cleanup(fail(BT_ERR_HIP, "sync"));
+  if (hipStreamBeginCapture(rho->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return cleanup(fail(BT_ERR_HIP, "bt_graph_create_dm: begin capture"));
+  int crc = BT_OK;
+  for (size_t i = 0; crc == BT_OK && i < narrow.size(); i++)
+    crc = launch_grid_pass(rho, (const unsigned char*)g->d_tab + i * tabsz, narrow[i] != 0, T, g->d_part);
+  const hipError_t ec = hipStreamEndCapture(rho->stream, &g->graph);
+  if (crc) return cleanup(crc);
+  if (ec != hipSuccess || !g->graph) return cleanup(fail(BT_ERR_HIP, "bt_graph_create_dm: end capture"));
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    return cleanup(fail(BT_ERR_HIP, "bt_graph_create_dm: instantiate"));
+  *out = g;
   return BT_OK;
 }
 

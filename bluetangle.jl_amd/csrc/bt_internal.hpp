@@ -110,6 +110,21 @@ int noalloc_setup(int device);
 // free the batch executor's cached state slab (unless a batch is running);
 // returns the bytes released
 size_t release_cached_memory();
+}  // namespace bt
+
+namespace btc {
+struct CircuitHost;
+}
+
+namespace bt {
+// density-matrix executor pieces shared with the graph capture; a grid
+// pass writes kGridAccRows rows of reduction partials per workgroup
+constexpr int kGridAccRows = 65;
+int plan_static_passes(bt_state_s* s, const btc::CircuitHost& ch, int T, std::vector<unsigned char>* tables,
+                       std::vector<uint8_t>* narrow, size_t* tabsz);
+int launch_grid_pass(bt_state_s* s, const void* dtab, bool narrow, int T, double* d_part);
+void grid_pass_lds_attr(int prec, int T);
+int dm_tile_bits(bt_state_s* rho);
 
 }  // namespace bt
 

@@ -251,6 +251,13 @@ int bt_autotune_fused(bt_state s, bt_circuit c, int* best_k, double* ms);
  * enqueued on s's stream; bt_state_sync waits).  The graph is bound to s
  * and owns its matrices.  complex64 with max_k = 5, or max_k <= 3. */
 int bt_graph_create_fused(bt_state s, bt_circuit c, int max_k, bt_graph* out);
+/* to_rho (ops.jl:806-844) captured as a HIP graph on density matrix rho: the
+ * density-matrix executor's passes (gates as E rho E^dagger, channels as
+ * one superoperator each), whose plan depends on nothing they compute,
+ * recorded once; bt_graph_launch replays the circuit on rho (reset rho with
+ * bt_state_set_basis first).  Circuits with measurements are rejected as in
+ * bt_run_circuit_dm. */
+int bt_graph_create_dm(bt_state rho, bt_circuit c, bt_graph* out);
 int bt_graph_launch(bt_graph g);
 int64_t bt_graph_passes(bt_graph g);
 int bt_graph_destroy(bt_graph g);

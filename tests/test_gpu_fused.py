@@ -145,3 +145,40 @@ def test_fusion_width_autotuner(bt, prec):
     c.run_fused(st, max_k=0)
     r, _, _, _, _ = O.run_sv(N, to_ref(spec), None)
     assert np.abs(st.to_numpy() - r).max() < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_density_matrix_graph(bt, prec):
+    """to_rho captured as one HIP graph (bt_graph_create_dm): every replay,
+    with eager work and synchronisations between the replays, equals the
+    eager density-matrix executor bit for bit and the oracle's rho; a circuit
+    with a measurement is rejected."""
+    from bluetangle_jl_amd.circuit import Circuit
+    from refcirc import random_kraus
+    rng = np.random.default_rng(77)
+    N = 6
+    spec = random_spec(N, 40, rng, kinds=("gate1", "gate2", "ccx", "chan1", "chan2"))
+    spec += [("kraus", random_kraus(1, rng, "general"), 2, -1), ("kraus", random_kraus(2, rng, "general"), 3, 5)]
+    spec += brickwork_spec(N, 3, 4, ad_gamma=0.05)
+    nm = ("depolarizing", 0.02, "amplitude_damping", 0.05)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm))
+    eager = bt.zero_rho(N, prec)
+    c.run_dm(eager)
+    want = eager.to_numpy()
+    ref = O.run_dm(N, to_ref(spec), O.ref_noise(*nm))
+    assert np.abs(want - ref).max() < 10 * TOL[prec]
+    rho = bt.zero_rho(N, prec)
+    g = c.capture_dm(rho)
+    assert g.passes > 0
+    other = bt.zero_state(N + 2, prec)
+    for rep in range(4):
+        rho.set_basis(0)
+        g.launch()
+        rho.sync()
+        assert np.array_equal(rho.to_numpy(), want), rep
+        bt.apply(other, bt.Op("H", 1 + rep % (N + 2)))  # eager work between replays
+        other.sync()
+    g.close()
+    cm = Circuit(to_bt([("gate", "H", 1, -1, -2, True), ("meas", "MZ", 1)], bt), N)
+    with pytest.raises(Exception):
+        cm.capture_dm(bt.zero_rho(N, prec))
