@@ -52,14 +52,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "noisy trajectories/s + achieved HBM GB/s per gate layer; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-# k_round's measured HBM traffic / algorithmic bytes on this circuit (rocprofv3
-# FETCH_SIZE + WRITE_SIZE, copy-calibrated, tiling 13,3,32); re-measured whenever
-# the kernel changes
-TRAFFIC_RATIO = 1.230
-# k_round's FP32 FMA flops per algorithmic byte on this circuit (rocprofv3
-# SQ_INSTS_VALU_FMA_F32 x 64 lanes x 2 over the same launches' algorithmic
-# bytes), and the non-packed FP32 vector peak it is measured against
-FMA_FLOP_PER_BYTE = 11.52
+# k_round's measured HBM traffic / algorithmic bytes on this circuit, by the
+# tiling's coalescing bits (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated,
+# tilings 13,3,16 and 13,4,16); a tiling not in the table reports no traffic
+TRAFFIC_RATIO = {3: 1.232, 4: 1.002}
+# k_round's FP32 FMA flops per algorithmic byte on this circuit at 3 coalescing
+# bits (rocprofv3 SQ_INSTS_VALU_FMA_F32 x 64 lanes x 2 over the same launches'
+# algorithmic bytes), and the non-packed FP32 vector peak it is measured against
+FMA_FLOP_PER_BYTE = {3: 11.52}
 VALU_FP32_PEAK_TFLOPS = 78.6
 
 
@@ -81,6 +81,13 @@ def c3_leg(bt, args):
     elif args.autotune > 0:  # untimed: pick tile bits / coalescing / tiles per workgroup on a sample
         tuning = c.autotune("c64", sample=args.autotune)
         log("autotune", tuning)
+    else:  # the library's defaults
+        import ctypes
+        from bluetangle_jl_amd import _lib as L
+        v = [ctypes.c_int(0) for _ in range(3)]
+        L.check(L.lib().bt_get_tuning(0, *[ctypes.byref(x) for x in v]))
+        tuning = {"tile_bits": v[0].value, "coalesce_bits": v[1].value, "tiles_per_block": v[2].value,
+                  "ms_per_traj": None}
     stats = []
     for w in range(args.warmup):
         c.run_batch(args.ntraj, seed=1000 + w, traj0=0, prec="c64", mode=args.mode)
@@ -583,6 +590,8 @@ def main():
     launches = sum(s["launches"] for s in stats)
     achieved = nbytes / (kms * 1e-3) / 1e9
     passes = sum(s["passes"] for s in stats) / (args.ntraj * args.steps)
+    cb = (tuning or {}).get("coalesce_bits")
+    ratio, fpb = TRAFFIC_RATIO.get(cb), FMA_FLOP_PER_BYTE.get(cb)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -606,16 +615,17 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_round<float>" if args.mode == "rounds" else "k_batch<float>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": round(nbytes / max(1, launches) * TRAFFIC_RATIO, 1),
-                     "traffic_source": "HBM read+write per launch = algorithmic x %.3f, the FETCH_SIZE/WRITE_SIZE "
-                                       "ratio measured for this kernel and circuit (profiles/k_round_traffic_pmc.txt)"
-                                       % TRAFFIC_RATIO,
+                     "traffic": round(nbytes / max(1, launches) * ratio, 1) if ratio else None,
+                     "traffic_source": ("HBM read+write per launch = algorithmic x %.3f, the FETCH_SIZE/WRITE_SIZE "
+                                        "ratio measured for this kernel, circuit and %d coalescing bits "
+                                        "(profiles/k_round_traffic_pmc.txt)" % (ratio, cb)) if ratio else
+                                       "not measured for %s coalescing bits" % cb,
                      "launches": int(launches), "avg_launch_ms": round(kms / max(1, launches), 4),
                      "bytes_per_launch": round(nbytes / max(1, launches), 1),
-                     "valu": {"achieved_tflops": round(achieved * FMA_FLOP_PER_BYTE / 1e3, 1),
+                     "valu": {"achieved_tflops": round(achieved * fpb / 1e3, 1) if fpb else None,
                               "peak_tflops": VALU_FP32_PEAK_TFLOPS,
-                              "frac": round(achieved * FMA_FLOP_PER_BYTE / 1e3 / VALU_FP32_PEAK_TFLOPS, 4),
-                              "flop_per_byte": FMA_FLOP_PER_BYTE,
+                              "frac": round(achieved * fpb / 1e3 / VALU_FP32_PEAK_TFLOPS, 4) if fpb else None,
+                              "flop_per_byte": fpb,
                               "note": "FP32 FMA work of the operators per algorithmic byte (rocprofv3 "
                                       "SQ_INSTS_VALU_FMA_F32 x 64 lanes x 2, profiles/k_round_c3_n22_4096traj.txt) "
                                       "against the non-packed FP32 vector peak (packed FP32 is not used)"}},
