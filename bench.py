@@ -67,6 +67,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def unfused_bytes_per_trajectory(cfg, elem=8):
+    """The per-op schedule's algorithmic bytes for one trajectory of a circuit:
+    every gate one read + write of the state (2S), every circuit-placed channel
+    and measurement a reduction read plus a read + write (3S), every noise
+    channel after a noisy gate 2S times the probability of a branch other than
+    the first (a multiple of the identity for depolarizing noise); S = bytes of
+    the state."""
+    S = float(1 << cfg["N"]) * elem
+    nm = cfg["noise"]
+    miss = {q: 1.0 - abs(ch.kraus[0][0, 0]) ** 2 if ch else 0.0 for q, ch in ((1, nm.q1), (2, nm.q2))} if nm else {}
+    total = 0.0
+    for o in cfg["ops"]:
+        if type(o).__name__ == "OpQC" or o.type == "\U0001f52c":
+            total += 3 * S
+        else:
+            total += 2 * S
+            if nm and o.noisy:
+                total += 2 * S * miss[1 if (o.q == 1 and o.control == -2) else 2]
+    return total
+
+
 def c3_leg(bt, args):
     from bluetangle_jl_amd import workloads as W
     from bluetangle_jl_amd.circuit import Circuit
@@ -591,6 +612,8 @@ def main():
     achieved = nbytes / (kms * 1e-3) / 1e9
     passes = sum(s["passes"] for s in stats) / (args.ntraj * args.steps)
     cb = (tuning or {}).get("coalesce_bits")
+    unfused = unfused_bytes_per_trajectory(cfg)
+    fused = nbytes / (args.ntraj * args.steps)
     ratio, fpb = TRAFFIC_RATIO.get(cb), FMA_FLOP_PER_BYTE.get(cb)
     out = {
         "metric": METRIC,
@@ -611,7 +634,15 @@ def main():
                    "ntraj": args.ntraj, "N": args.N, "layers": args.layers, "executor": args.mode,
                    "passes_per_trajectory": round(passes, 3),
                    "boundary_draws": int(sum(s["boundary_draws"] for s in stats)),
-                   "tuning": tuning},
+                   "tuning": tuning,
+                   "unfused_equivalent": {
+                       "gbs": round(value * unfused / 1e9, 1),
+                       "bytes_per_trajectory": round(unfused, 1),
+                       "fused_bytes_per_trajectory": round(fused, 1),
+                       "reduction": round(unfused / fused, 1),
+                       "note": "an equivalent, not a measured rate: the per-op schedule's algorithmic bytes "
+                               "(gate 2S, amplitude damping and MZ 3S, noise branches 2S x their probability; "
+                               "S = the complex64 state's bytes) per trajectory x trajectories/s"}},
         "roofline": {"bound": "hbm", "kernel": "k_round<float>" if args.mode == "rounds" else "k_batch<float>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
