@@ -33,25 +33,24 @@ __device__ __forceinline__ double prob(const C<R>* s, uint64_t i) {
   return (double)v.x * (double)v.x + (double)v.y * (double)v.y;
 }
 
+// One wavefront per chunk (no workgroup barriers): lane l sums entries
+// l, l + 64, ... of the chunk in order, then a fixed butterfly over the
+// lanes.  The workgroup's four wavefronts take four consecutive chunks.
 template <typename R>
 __global__ void __launch_bounds__(256) k_chunk_sums(const C<R>* s, uint64_t n, uint64_t nchunks, double* sums) {
-  __shared__ double red[256];
-  for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    const uint64_t base = c * kChunk;
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * 4;
+  for (uint64_t c = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); c < nchunks; c += nw) {
+    const uint64_t base = c * kChunk + (uint64_t)lane;
     double t = 0.0;
 #pragma unroll
-    for (int j = 0; j < kChunk / 256; j++) {
-      const uint64_t i = base + threadIdx.x + 256u * j;
+    for (int j = 0; j < kChunk / 64; j++) {
+      const uint64_t i = base + 64u * j;
       if (i < n) t += prob<R>(s, i);
     }
-    red[threadIdx.x] = t;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) sums[c] = red[0];
-    __syncthreads();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == 0) sums[c] = t;
   }
 }
 
@@ -170,7 +169,7 @@ int launch_sample(const void* state, int prec, int nbits, hipStream_t st, void* 
   const uint64_t nchunks = (n + kChunk - 1) / kChunk;
   double* sums = (double*)scratch;
   double* prefix = sums + nchunks;
-  const unsigned g1 = (unsigned)std::min<uint64_t>(nchunks, 1u << 16);
+  const unsigned g1 = (unsigned)std::min<uint64_t>((nchunks + 3) / 4, 1u << 16);
   const unsigned g3 = (unsigned)((shots + 3) / 4);
   if (prec == BT_C64) {
     hipLaunchKernelGGL(k_chunk_sums<float>, dim3(g1), dim3(256), 0, st, (const C<float>*)state, n, nchunks, sums);
