@@ -477,8 +477,17 @@ void grid_pass_lds_attr(int prec, int T) {
 
 // Tile bits of the density-matrix executor: a superoperator spans up to
 // kMaxQK bits of vec(rho), so a smaller tile setting is raised to hold it.
+// Tile bits of the density-matrix executor.  Small complex64 density
+// matrices (vec(rho) of at most 2^20 entries, N <= 10) run on tiles of at
+// most 2^12 entries: half the LDS per workgroup, so twice the workgroups per
+// CU and per state, for the 16 x 16 superoperators applied in LDS (measured
+// on noisy brickworks, 16 layers: N = 8 2.99 vs 3.37 ms, N = 10 3.94 vs
+// 4.57 ms per circuit; from N = 11 up 2^13-entry tiles are faster).  A
+// smaller tile setting is kept.
 int dm_tile_bits(bt_state_s* rho) {
-  return std::max(tile_bits_for(rho->prec, rho->nbits), std::min(rho->nbits, bti::kMaxQK));
+  int T = tile_bits_for(rho->prec, rho->nbits);
+  if (rho->prec == BT_C64 && rho->nbits <= 20) T = std::min(T, 12);
+  return std::max(T, std::min(rho->nbits, bti::kMaxQK));
 }
 
 }  // namespace bt
