@@ -215,6 +215,7 @@ int bt_state_destroy(bt_state s) {
   if (s->d_out) (void)hipFree(s->d_out);
   if (s->d_mat) (void)hipFree(s->d_mat);
   if (s->h_out) (void)hipHostFree(s->h_out);
+  if (s->grid) bt::free_grid_res(s->grid);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return BT_OK;

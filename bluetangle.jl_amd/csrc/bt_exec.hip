@@ -241,6 +241,59 @@ static unsigned grid_pass_blocks(uint64_t ntiles) {
   return (unsigned)std::min<uint64_t>(ntiles, (uint64_t)cus * 8);
 }
 
+constexpr int kRing = 8;  // pass tables in flight
+
+// The circuit executor's staging of a state, made on its first run and kept
+// until the state is destroyed (a state is used by one host thread at a
+// time): pinned and device rings of pass tables, reduction partials for the
+// widest grid, the finished sums and their pinned copy, events.
+struct GridRes {
+  void* h_ring = nullptr;
+  void* d_ring = nullptr;
+  double* d_part = nullptr;
+  double* d_acc = nullptr;
+  double* h_acc = nullptr;
+  hipEvent_t ev[kRing] = {};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+void free_grid_res(GridRes* g) {
+  if (!g) return;
+  for (int i = 0; i < kRing; i++)
+    if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+  if (g->e0) (void)hipEventDestroy(g->e0);
+  if (g->e1) (void)hipEventDestroy(g->e1);
+  if (g->h_ring) (void)hipHostFree(g->h_ring);
+  if (g->h_acc) (void)hipHostFree(g->h_acc);
+  if (g->d_ring) (void)hipFree(g->d_ring);
+  if (g->d_part) (void)hipFree(g->d_part);
+  if (g->d_acc) (void)hipFree(g->d_acc);
+  delete g;
+}
+
+static int grid_res(bt_state_s* s, GridRes** out) {
+  if (!s->grid) {
+    const size_t tabsz = s->prec == BT_C64 ? sizeof(GridTable<float>) : sizeof(GridTable<double>);
+    const int cus = ctx().cus > 0 ? ctx().cus : 256;
+    auto* g = new GridRes();
+    bool ok = hipHostMalloc(&g->h_ring, tabsz * kRing) == hipSuccess &&
+              hipMalloc(&g->d_ring, tabsz * kRing) == hipSuccess &&
+              hipMalloc((void**)&g->d_part, sizeof(double) * kAcc * (size_t)cus * 8) == hipSuccess &&
+              hipMalloc((void**)&g->d_acc, sizeof(double) * kAcc) == hipSuccess &&
+              hipHostMalloc((void**)&g->h_acc, sizeof(double) * kAcc) == hipSuccess;
+    for (int i = 0; i < kRing && ok; i++) ok = hipEventCreateWithFlags(&g->ev[i], hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreate(&g->e0) == hipSuccess && hipEventCreate(&g->e1) == hipSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      free_grid_res(g);
+      return fail(BT_ERR_NOMEM, "run_grid: allocation failed");
+    }
+    s->grid = g;
+  }
+  *out = s->grid;
+  return BT_OK;
+}
+
 // Host driver of the interpreter over grid passes on state s (SV or DM).
 static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, uint64_t traj, const double* uni,
                     int64_t nuni, int32_t* outs, int64_t outcap, int64_t* nout, bt_run_stats* stats, int fold,
@@ -265,39 +318,18 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
   const uint64_t ntiles = 1ull << (nbits - std::min(T, nbits));
   const unsigned blocks = grid_pass_blocks(ntiles);
   const size_t lds = (size_t)(1u << std::min(T, nbits)) * s->elem();
-  constexpr int kRing = 8;
-  void* h_ring = nullptr;
-  void* d_ring = nullptr;
-  double* d_part = nullptr;
-  double* d_acc = nullptr;
-  double* h_acc = nullptr;
-  hipEvent_t ev[kRing];
-  hipEvent_t e0, e1;
-  int rc = BT_OK;
-  for (int i = 0; i < kRing; i++) ev[i] = nullptr;
-  BT_HIP(hipHostMalloc(&h_ring, tabsz * kRing));
-  if (hipMalloc(&d_ring, tabsz * kRing) != hipSuccess || hipMalloc(&d_part, sizeof(double) * kAcc * blocks) ||
-      hipMalloc(&d_acc, sizeof(double) * kAcc) || hipHostMalloc(&h_acc, sizeof(double) * kAcc)) {
-    rc = fail(BT_ERR_NOMEM, "run_grid: allocation failed");
-  }
-  for (int i = 0; i < kRing && rc == BT_OK; i++)
-    if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) rc = fail(BT_ERR_HIP, "event");
-  if (rc == BT_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
-    rc = fail(BT_ERR_HIP, "event");
-  if (rc == BT_OK) {
-    if (s->prec == BT_C64) {
-      (void)hipFuncSetAttribute((const void*)k_grid_pass<float, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-      (void)hipFuncSetAttribute((const void*)k_grid_pass<float, uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-    } else {
-      (void)hipFuncSetAttribute((const void*)k_grid_pass<double, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-      (void)hipFuncSetAttribute((const void*)k_grid_pass<double, uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-    }
-    (void)hipEventRecord(e0, s->stream);
-  }
+  GridRes* gr = nullptr;
+  int rc = grid_res(s, &gr);
+  if (rc) return rc;
+  void* const h_ring = gr->h_ring;
+  void* const d_ring = gr->d_ring;
+  double* const d_part = gr->d_part;
+  double* const d_acc = gr->d_acc;
+  double* const h_acc = gr->h_acc;
+  hipEvent_t* const ev = gr->ev;
+  hipEvent_t const e0 = gr->e0, e1 = gr->e1;
+  grid_pass_lds_attr(s->prec, std::min(T, nbits));
+  (void)hipEventRecord(e0, s->stream);
   Plan plan;
   PassResult res;
   memset(&res, 0, sizeof res);
@@ -309,8 +341,9 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
       rc = fail(BT_ERR_STATE, "run_grid: pass limit exceeded");
       break;
     }
+    // the slot's previous table copy (this run's or an earlier one's) is done
     const int slot = (int)(npass % kRing);
-    if (npass > kRing && hipEventSynchronize(ev[slot]) != hipSuccess) {
+    if (hipEventSynchronize(ev[slot]) != hipSuccess) {
       rc = fail(BT_ERR_HIP, "run_grid: event sync");
       break;
     }
@@ -357,7 +390,10 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
       rc = fail(BT_ERR_HIP, "run_grid: pass launch");
       break;
     }
-    if (plan.aligned || plan.red != bti::RED_NONE) {
+    // the host waits for a pass's sums only when a decision needs them or
+    // the norm is folded into the next pass (fold = 0, a density matrix:
+    // passes follow one another without a round trip)
+    if (plan.red != bti::RED_NONE || (plan.aligned && fold)) {
       hipLaunchKernelGGL(k_finalize_acc, dim3(kAcc), dim3(256), 0, s->stream, (const double*)d_part, (int)blocks,
                          d_acc);
       if (hipMemcpyAsync(h_acc, d_acc, sizeof(double) * kAcc, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
@@ -387,15 +423,6 @@ static int run_grid(bt_state_s* s, const btc::CircuitHost& ch, uint64_t seed, ui
     if (rc == BT_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) stats->kernel_ms = ms;
   }
   if (nout) *nout = ip->nout;
-  for (int i = 0; i < kRing; i++)
-    if (ev[i]) (void)hipEventDestroy(ev[i]);
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  if (h_ring) (void)hipHostFree(h_ring);
-  if (h_acc) (void)hipHostFree(h_acc);
-  if (d_ring) (void)hipFree(d_ring);
-  if (d_part) (void)hipFree(d_part);
-  if (d_acc) (void)hipFree(d_acc);
   return rc;
 }
 
@@ -463,6 +490,43 @@ int launch_grid_pass(bt_state_s* s, const void* dtab, bool narrow, int T, double
   return BT_OK;
 }
 
+// Passes whose plan depends on nothing they compute (a density matrix) run
+// eagerly from tables planned on the host beforehand: uploaded in one copy
+// and launched back to back with no host round trip between them.
+static int run_static(bt_state_s* s, const std::vector<unsigned char>& tables, const std::vector<uint8_t>& narrow,
+                      size_t tabsz, int T, bt_run_stats* stats) {
+  GridRes* gr = nullptr;
+  int rc = grid_res(s, &gr);
+  if (rc) return rc;
+  void* d_tab = nullptr;
+  if (!tables.empty() && hipMalloc(&d_tab, tables.size()) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(BT_ERR_NOMEM, "density-matrix run: pass tables");
+  }
+  if (d_tab && hipMemcpyAsync(d_tab, tables.data(), tables.size(), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    rc = fail(BT_ERR_HIP, "density-matrix run: table upload");
+  grid_pass_lds_attr(s->prec, std::min(T, s->nbits));
+  if (rc == BT_OK) (void)hipEventRecord(gr->e0, s->stream);
+  for (size_t i = 0; rc == BT_OK && i < narrow.size(); i++)
+    rc = launch_grid_pass(s, (const unsigned char*)d_tab + i * tabsz, narrow[i] != 0, T, gr->d_part);
+  if (rc == BT_OK) {
+    (void)hipEventRecord(gr->e1, s->stream);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(BT_ERR_HIP, "density-matrix run: sync");
+  } else {
+    (void)hipStreamSynchronize(s->stream);  // the upload is done before the tables are freed
+  }
+  if (d_tab) (void)hipFree(d_tab);
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    stats->passes = (int64_t)narrow.size();
+    stats->bytes = 2.0 * (double)narrow.size() * (double)(1ull << s->nbits) * (double)s->elem();
+    stats->launches = (int64_t)narrow.size();
+    float ms = 0;
+    if (rc == BT_OK && hipEventElapsedTime(&ms, gr->e0, gr->e1) == hipSuccess) stats->kernel_ms = ms;
+  }
+  return rc;
+}
+
 // Allow the grid pass's dynamic LDS tile of 2^T elements.
 void grid_pass_lds_attr(int prec, int T) {
   const int lds = (int)((size_t)(1u << T) * (prec == BT_C64 ? 8 : 16));
@@ -477,7 +541,7 @@ void grid_pass_lds_attr(int prec, int T) {
 
 // Tile bits of the density-matrix executor: a superoperator spans up to
 // kMaxQK bits of vec(rho), so a smaller tile setting is raised to hold it.
-// Tile bits of the density-matrix executor.  Small complex64 density
+// Small complex64 density
 // matrices (vec(rho) of at most 2^20 entries, N <= 10) run on tiles of at
 // most 2^12 entries: half the LDS per workgroup, so twice the workgroups per
 // CU and per state, for the 16 x 16 superoperators applied in LDS (measured
@@ -543,12 +607,31 @@ int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats) {
   if (rc) return rc;
   if (rho->kind != BT_DM) return fail(BT_ERR_ARG, "bt_run_circuit_dm needs a density matrix");
   if (rho->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
-  btc::CircuitHost dm;
-  std::string msg;
-  rc = lower_dm(c->h, &dm, &msg);
-  if (rc) return fail(rc, msg);
-  int64_t nout = 0;
-  return run_grid(rho, dm, 0, 0, nullptr, 0, nullptr, 0, &nout, stats, 0, dm_tile_bits(rho));
+  // the plan of this circuit at this tiling, planned once and kept on the circuit
+  const int T = dm_tile_bits(rho), cb = coalesce_bits(rho->prec);
+  std::vector<unsigned char> tables;
+  std::vector<uint8_t> narrow;
+  size_t tabsz = 0;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    bt_circuit_s::DmPlan& P = c->dm_plan;
+    if (P.T != T || P.prec != rho->prec || P.cb != cb) {
+      btc::CircuitHost dm;
+      std::string msg;
+      rc = lower_dm(c->h, &dm, &msg);
+      if (rc) return fail(rc, msg);
+      P.T = -1;
+      rc = plan_static_passes(rho, dm, T, &P.tables, &P.narrow, &P.tabsz);
+      if (rc) return rc;
+      P.T = T;
+      P.prec = rho->prec;
+      P.cb = cb;
+    }
+    tables = P.tables;
+    narrow = P.narrow;
+    tabsz = P.tabsz;
+  }
+  return run_static(rho, tables, narrow, tabsz, T, stats);
 }
 
 int bt_run_batch_persistent(bt_circuit c, int prec, uint64_t seed, int64_t traj0, int64_t ntraj, int* outcomes, int64_t outcap,

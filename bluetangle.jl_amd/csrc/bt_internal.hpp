@@ -128,6 +128,11 @@ int dm_tile_bits(bt_state_s* rho);
 
 }  // namespace bt
 
+namespace bt {
+struct GridRes;  // the circuit executor's staging of a state (bt_exec.hip)
+void free_grid_res(GridRes* g);
+}  // namespace bt
+
 struct bt_state_s {
   int n = 0;         // qubits
   int kind = BT_SV;  // BT_SV / BT_DM
@@ -139,6 +144,7 @@ struct bt_state_s {
   double* d_out = nullptr;   // reduction result (device)
   double* h_out = nullptr;   // pinned host copy of the result
   void* d_mat = nullptr;     // device matrix staging (k >= 4 blocks)
+  bt::GridRes* grid = nullptr;  // circuit executor staging, made on the first run, kept with the state
   hipStream_t stream = nullptr;
   size_t elem() const { return prec == BT_C64 ? 8 : 16; }
 };
