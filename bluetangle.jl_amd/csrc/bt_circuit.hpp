@@ -369,13 +369,13 @@ struct bt_circuit_s {
   double* d_chanw = nullptr;
   std::mutex mu;
   int fused_k = -1;  // bt_autotune_fused's choice (0: register-tile executor), -1: not tuned
-  // the density-matrix executor's pass tables of this circuit (their plan
-  // depends on nothing the passes compute), kept for the tiling they were
-  // planned with; guarded by mu
-  struct DmPlan {
+  // pass tables of this circuit whose plan depends on nothing the passes
+  // compute (its density-matrix form; its state-vector form when it holds
+  // gates only), kept for the tiling they were planned with; guarded by mu
+  struct StaticPlan {
     int T = -1, prec = -1, cb = -1;
     size_t tabsz = 0;
     std::vector<unsigned char> tables;
     std::vector<uint8_t> narrow;
-  } dm_plan;
+  } dm_plan, sv_plan;
 };

@@ -490,9 +490,31 @@ int launch_grid_pass(bt_state_s* s, const void* dtab, bool narrow, int T, double
   return BT_OK;
 }
 
-// Passes whose plan depends on nothing they compute (a density matrix) run
-// eagerly from tables planned on the host beforehand: uploaded in one copy
-// and launched back to back with no host round trip between them.
+// The pass tables of circuit form ch (c's own or its density-matrix form)
+// at tile bits T on states like s: planned once per tiling and kept in the
+// circuit's slot P, copied out for this run.
+static int static_plan(bt_state_s* s, bt_circuit_s* c, const btc::CircuitHost& ch, bt_circuit_s::StaticPlan* P, int T,
+                       std::vector<unsigned char>* tables, std::vector<uint8_t>* narrow, size_t* tabsz) {
+  const int cb = coalesce_bits(s->prec);
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (P->T != T || P->prec != s->prec || P->cb != cb) {
+    P->T = -1;
+    const int rc = plan_static_passes(s, ch, T, &P->tables, &P->narrow, &P->tabsz);
+    if (rc) return rc;
+    P->T = T;
+    P->prec = s->prec;
+    P->cb = cb;
+  }
+  *tables = P->tables;
+  *narrow = P->narrow;
+  *tabsz = P->tabsz;
+  return BT_OK;
+}
+
+// Passes whose plan depends on nothing they compute (a density matrix, a
+// gate-only state-vector circuit) run eagerly from tables planned on the
+// host beforehand: uploaded in one copy and launched back to back with no
+// host round trip between them.
 static int run_static(bt_state_s* s, const std::vector<unsigned char>& tables, const std::vector<uint8_t>& narrow,
                       size_t tabsz, int T, bt_run_stats* stats) {
   GridRes* gr = nullptr;
@@ -597,8 +619,21 @@ int bt_run_circuit(bt_state s, bt_circuit c, uint64_t seed, uint64_t traj, const
   if (s->kind != BT_SV) return fail(BT_ERR_ARG, "bt_run_circuit needs a state vector (use bt_run_circuit_dm)");
   if (s->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
   if (uniforms && nuni < c->h.max_draws) return fail(BT_ERR_DRAWS, "too few uniforms for the circuit");
-  return run_grid(s, c->h, seed, traj, uniforms, nuni, outcomes, outcap, nout, stats, 1,
-                  tile_bits_for(s->prec, s->nbits));
+  bool gates_only = true;
+  for (int i = 0; i < c->h.ntop && gates_only; i++) gates_only = c->h.ops[i].kind == BT_OP_GATE;
+  if (!gates_only)
+    return run_grid(s, c->h, seed, traj, uniforms, nuni, outcomes, outcap, nout, stats, 1,
+                    tile_bits_for(s->prec, s->nbits));
+  // gates only: no decisions and no renormalisation, so the passes are
+  // planned once per tiling and launched back to back
+  const int T = tile_bits_for(s->prec, s->nbits);
+  std::vector<unsigned char> tables;
+  std::vector<uint8_t> narrow;
+  size_t tabsz = 0;
+  rc = static_plan(s, c, c->h, &c->sv_plan, T, &tables, &narrow, &tabsz);
+  if (rc) return rc;
+  if (nout) *nout = 0;
+  return run_static(s, tables, narrow, tabsz, T, stats);
 }
 
 int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats) {
@@ -607,30 +642,16 @@ int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats) {
   if (rc) return rc;
   if (rho->kind != BT_DM) return fail(BT_ERR_ARG, "bt_run_circuit_dm needs a density matrix");
   if (rho->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
-  // the plan of this circuit at this tiling, planned once and kept on the circuit
-  const int T = dm_tile_bits(rho), cb = coalesce_bits(rho->prec);
+  btc::CircuitHost dm;
+  std::string msg;
+  rc = lower_dm(c->h, &dm, &msg);
+  if (rc) return fail(rc, msg);
+  const int T = dm_tile_bits(rho);
   std::vector<unsigned char> tables;
   std::vector<uint8_t> narrow;
   size_t tabsz = 0;
-  {
-    std::lock_guard<std::mutex> lk(c->mu);
-    bt_circuit_s::DmPlan& P = c->dm_plan;
-    if (P.T != T || P.prec != rho->prec || P.cb != cb) {
-      btc::CircuitHost dm;
-      std::string msg;
-      rc = lower_dm(c->h, &dm, &msg);
-      if (rc) return fail(rc, msg);
-      P.T = -1;
-      rc = plan_static_passes(rho, dm, T, &P.tables, &P.narrow, &P.tabsz);
-      if (rc) return rc;
-      P.T = T;
-      P.prec = rho->prec;
-      P.cb = cb;
-    }
-    tables = P.tables;
-    narrow = P.narrow;
-    tabsz = P.tabsz;
-  }
+  rc = static_plan(rho, c, dm, &c->dm_plan, T, &tables, &narrow, &tabsz);
+  if (rc) return rc;
   return run_static(rho, tables, narrow, tabsz, T, stats);
 }
 
