@@ -117,6 +117,38 @@ def test_run_circuit_dm_random(bt, tiles, prec):
         assert np.abs(rho.to_numpy() - ref).max() < TOL[prec], (T, seed)
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_kept_plans_follow_the_tiling(bt, tiles, prec):
+    """A density-matrix run and a gate-only state-vector run keep their pass
+    plans on the circuit (and the executor's staging on the state); changing
+    the tiling plans them again: one circuit and one state run at tile
+    settings 0, 4, 9 and 0 again match the oracle every time, and the pass
+    count follows the tiling."""
+    from bluetangle_jl_amd.circuit import Circuit
+    rng = np.random.default_rng(480)
+    N = 6
+    spec = random_spec(N, 50, rng, kinds=("gate1", "gate2", "ccx", "chan1", "chan2"))
+    nm_args = ("depolarizing", 0.05, "amplitude_damping", 0.1)
+    c = Circuit(to_bt(spec, bt), N, bt.NoiseModel(*nm_args))
+    ref = O.run_dm(N, to_ref(spec), O.ref_noise(*nm_args))
+    rho = bt.zero_rho(N, prec)
+    G = 9
+    gspec = random_spec(G, 60, rng, kinds=("gate1", "gate2", "ccx"))
+    g = Circuit(to_bt(gspec, bt), G, False)
+    gref, _, _, _, _ = O.run_sv(G, to_ref(gspec), None, O.uniforms(0, 0, 16), tol=1e-6)
+    psi = bt.zero_state(G, prec)
+    passes = {}
+    for T in (0, 4, 9, 0):
+        tiles(T)
+        st = c.run_dm(rho.set_basis(0))
+        assert np.abs(rho.to_numpy() - ref).max() < TOL[prec], T
+        _, sg = g.run_on(psi.set_basis(0))
+        assert np.abs(psi.to_numpy() - gref).max() < TOL[prec], T
+        passes.setdefault(T, []).append((st["passes"], sg["passes"]))
+    assert passes[0][0] == passes[0][1]
+    assert passes[4][0][0] > passes[9][0][0] and passes[4][0][1] > passes[9][0][1]
+
+
 def test_c2_density_matrix_config(bt):
     """Config 2: rho N=8, brickwork 16 layers, depolarizing 1e-3 / 1e-2 and
     amplitude damping 5e-3 per qubit per layer; complex64 rho vs the
