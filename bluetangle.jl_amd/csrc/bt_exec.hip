@@ -570,11 +570,13 @@ void grid_pass_lds_attr(int prec, int T) {
 // on noisy brickworks, 16 layers: N = 8 2.99 vs 3.37 ms, N = 10 3.94 vs
 // 4.57 ms per circuit; from N = 11 up 2^13-entry tiles are faster).  A
 // smaller tile setting is kept.
-int dm_tile_bits(bt_state_s* rho) {
-  int T = tile_bits_for(rho->prec, rho->nbits);
-  if (rho->prec == BT_C64 && rho->nbits <= 20) T = std::min(T, 12);
-  return std::max(T, std::min(rho->nbits, bti::kMaxQK));
+int dm_tile_bits_for(int prec, int nbits) {
+  int T = tile_bits_for(prec, nbits);
+  if (prec == BT_C64 && nbits <= 20) T = std::min(T, 12);
+  return std::max(T, std::min(nbits, bti::kMaxQK));
 }
+
+int dm_tile_bits(bt_state_s* rho) { return dm_tile_bits_for(rho->prec, rho->nbits); }
 
 }  // namespace bt
 

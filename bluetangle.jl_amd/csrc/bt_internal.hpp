@@ -124,7 +124,8 @@ int plan_static_passes(bt_state_s* s, const btc::CircuitHost& ch, int T, std::ve
                        std::vector<uint8_t>* narrow, size_t* tabsz);
 int launch_grid_pass(bt_state_s* s, const void* dtab, bool narrow, int T, double* d_part);
 void grid_pass_lds_attr(int prec, int T);
-int dm_tile_bits(bt_state_s* rho);
+int dm_tile_bits(bt_state_s* rho);              // tile bits of a density matrix's passes
+int dm_tile_bits_for(int prec, int nbits);      // the same for vec(rho) of nbits index bits
 
 }  // namespace bt
 

@@ -289,7 +289,7 @@ class RoundEngine {
                       j->circ->chanw.data(), j->nbits};
     // density-matrix jobs (no norm folding): a superoperator spans up to
     // kMaxQK bits of vec(rho), so a smaller tile setting is raised to hold it
-    const int T = j->fold ? tile_T(j->nbits) : std::max(tile_T(j->nbits), std::min(j->nbits, bti::kMaxQK));
+    const int T = j->fold ? tile_T(j->nbits) : dm_tile_bits_for(prec_, j->nbits);
     j->ip->init(prog, j->slots.data(), T, coalesce_bits(prec_));
     j->ip->seed = j->seed;
     j->ip->traj = j->traj;
