@@ -199,8 +199,27 @@ def gate_layer_leg(bt, args):
             bt.apply(st, op)
         st.sync()
         per_bit.append(4 * 2.0 * (1 << N) * 8 / (time.perf_counter() - t1) / 1e9)
+    # two-qubit gates: a random 4x4 unitary on each adjacent bit pair (b, b+1)
+    import numpy as np
+    rng = np.random.default_rng(2)
+    U2, _ = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))
+    per_pair = []
+    for b in range(N - 1):
+        op = bt.Op("U2", U2, N - b - 1, N - b)
+        bt.apply(st, op)
+        st.sync()
+        t1 = time.perf_counter()
+        for _ in range(4):
+            bt.apply(st, op)
+        st.sync()
+        per_pair.append(4 * 2.0 * (1 << N) * 8 / (time.perf_counter() - t1) / 1e9)
     obs = observables_on(bt, st, N)
     del st
+    two = dict(per_pair_gbs=[round(x, 1) for x in per_pair],
+               mean_frac=round(sum(per_pair) / len(per_pair) / HBM_PEAK_GBS, 4),
+               min_frac=round(min(per_pair) / HBM_PEAK_GBS, 4),
+               min_pair=int(min(range(N - 1), key=lambda i: per_pair[i])),
+               note="a random two-qubit unitary on bits (b, b+1), four launches per pair; per_pair_gbs[b]")
     return dict(N=N, dtype="complex64", layers=nlayers, ms_per_layer=round(dt / nlayers * 1e3, 4),
                 achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
                 per_bit_gbs=[round(x, 1) for x in per_bit], per_bit_min_frac=round(min(per_bit) / HBM_PEAK_GBS, 4),
@@ -208,7 +227,7 @@ def gate_layer_leg(bt, args):
                 per_bit_min_bit=int(min(range(N), key=lambda i: per_bit[i])),
                 note="one Hadamard per layer on each qubit in turn; read+write of the state per layer; wall time "
                      "of back-to-back launches; per_bit_gbs[b]: four launches on target bit b alone",
-                observables=obs)
+                two_qubit=two, observables=obs)
 
 
 def observables_on(bt, st, N):

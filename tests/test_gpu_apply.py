@@ -126,6 +126,34 @@ def test_large_state_gate_roundtrip(bt, bit):
     assert abs(st.norm2() - 1) < 1e-5
 
 
+@pytest.mark.parametrize("bits", [(1, 2), (9, 10), (12, 20), (23, 24), (24, 25), (3, 22)])
+def test_large_state_two_qubit_gates(bt, bits):
+    """N=26 complex64: a random two-qubit unitary on bit pairs on both sides
+    of the middle range (where the per-op kernel takes two groups per lane)
+    matches a dense host application to 1e-6, and U then U^dagger restores
+    the state."""
+    N = 26
+    b1, b2 = bits
+    rng = np.random.default_rng(300 + b1 + 40 * b2)
+    v = (rng.normal(size=2 ** N) + 1j * rng.normal(size=2 ** N)).astype(np.complex64)
+    v /= np.linalg.norm(v)
+    a = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+    U, _ = np.linalg.qr(a)
+    # the reference's first factor acts on `qubit`: bit b2 (higher) carries matrix index bit 1
+    q1, q2 = N - b2, N - b1
+    st = bt.from_numpy(v.astype(np.complex128), "c64")
+    bt.apply(st, bt.Op("u2", U, q1, q2))
+    t = v.astype(np.complex128).reshape([2] * N)  # axis i = bit N - 1 - i
+    ax1, ax2 = N - 1 - b2, N - 1 - b1
+    t = np.moveaxis(t, (ax1, ax2), (0, 1)).reshape(4, -1)
+    t = (U @ t).reshape([2, 2] + [2] * (N - 2))
+    r = np.moveaxis(t, (0, 1), (ax1, ax2)).reshape(-1)
+    got = st.to_numpy()
+    assert np.abs(got - r).max() < 1e-6, np.abs(got - r).max()
+    bt.apply(st, bt.Op("u2d", U.conj().T, q1, q2))
+    assert np.abs(st.to_numpy() - v).max() < 1e-6
+
+
 def test_errors_are_reported(bt):
     st = bt.zero_state(3)
     with pytest.raises((ValueError, bt.BTError)):
