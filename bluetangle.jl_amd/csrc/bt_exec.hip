@@ -490,16 +490,26 @@ int launch_grid_pass(bt_state_s* s, const void* dtab, bool narrow, int T, double
   return BT_OK;
 }
 
-// The pass tables of circuit form ch (c's own or its density-matrix form)
-// at tile bits T on states like s: planned once per tiling and kept in the
-// circuit's slot P, copied out for this run.
-static int static_plan(bt_state_s* s, bt_circuit_s* c, const btc::CircuitHost& ch, bt_circuit_s::StaticPlan* P, int T,
-                       std::vector<unsigned char>* tables, std::vector<uint8_t>* narrow, size_t* tabsz) {
+// The pass tables of circuit c (dm: its density-matrix form) at tile bits T
+// on states like s: planned once per tiling and kept in the circuit's slot,
+// copied out for this run.
+static int static_plan(bt_state_s* s, bt_circuit_s* c, bool dm, int T, std::vector<unsigned char>* tables,
+                       std::vector<uint8_t>* narrow, size_t* tabsz) {
   const int cb = coalesce_bits(s->prec);
   std::lock_guard<std::mutex> lk(c->mu);
+  bt_circuit_s::StaticPlan* P = dm ? &c->dm_plan : &c->sv_plan;
   if (P->T != T || P->prec != s->prec || P->cb != cb) {
     P->T = -1;
-    const int rc = plan_static_passes(s, ch, T, &P->tables, &P->narrow, &P->tabsz);
+    int rc;
+    if (dm) {
+      btc::CircuitHost lowered;
+      std::string msg;
+      rc = lower_dm(c->h, &lowered, &msg);
+      if (rc) return fail(rc, msg);
+      rc = plan_static_passes(s, lowered, T, &P->tables, &P->narrow, &P->tabsz);
+    } else {
+      rc = plan_static_passes(s, c->h, T, &P->tables, &P->narrow, &P->tabsz);
+    }
     if (rc) return rc;
     P->T = T;
     P->prec = s->prec;
@@ -523,17 +533,17 @@ static int run_static(bt_state_s* s, const std::vector<unsigned char>& tables, c
   void* d_tab = nullptr;
   if (!tables.empty() && hipMalloc(&d_tab, tables.size()) != hipSuccess) {
     (void)hipGetLastError();
-    return fail(BT_ERR_NOMEM, "density-matrix run: pass tables");
+    return fail(BT_ERR_NOMEM, "run: pass tables");
   }
   if (d_tab && hipMemcpyAsync(d_tab, tables.data(), tables.size(), hipMemcpyHostToDevice, s->stream) != hipSuccess)
-    rc = fail(BT_ERR_HIP, "density-matrix run: table upload");
+    rc = fail(BT_ERR_HIP, "run: table upload");
   grid_pass_lds_attr(s->prec, std::min(T, s->nbits));
   if (rc == BT_OK) (void)hipEventRecord(gr->e0, s->stream);
   for (size_t i = 0; rc == BT_OK && i < narrow.size(); i++)
     rc = launch_grid_pass(s, (const unsigned char*)d_tab + i * tabsz, narrow[i] != 0, T, gr->d_part);
   if (rc == BT_OK) {
     (void)hipEventRecord(gr->e1, s->stream);
-    if (hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(BT_ERR_HIP, "density-matrix run: sync");
+    if (hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(BT_ERR_HIP, "run: sync");
   } else {
     (void)hipStreamSynchronize(s->stream);  // the upload is done before the tables are freed
   }
@@ -563,13 +573,12 @@ void grid_pass_lds_attr(int prec, int T) {
 
 // Tile bits of the density-matrix executor: a superoperator spans up to
 // kMaxQK bits of vec(rho), so a smaller tile setting is raised to hold it.
-// Small complex64 density
-// matrices (vec(rho) of at most 2^20 entries, N <= 10) run on tiles of at
-// most 2^12 entries: half the LDS per workgroup, so twice the workgroups per
-// CU and per state, for the 16 x 16 superoperators applied in LDS (measured
-// on noisy brickworks, 16 layers: N = 8 2.99 vs 3.37 ms, N = 10 3.94 vs
-// 4.57 ms per circuit; from N = 11 up 2^13-entry tiles are faster).  A
-// smaller tile setting is kept.
+// Small complex64 density matrices (vec(rho) of at most 2^20 entries,
+// N <= 10) run on tiles of at most 2^12 entries: half the LDS per
+// workgroup, so twice the workgroups per CU and per state, for the 16 x 16
+// superoperators applied in LDS (measured on noisy brickworks, 16 layers:
+// N = 8 2.99 vs 3.37 ms, N = 10 3.94 vs 4.57 ms per circuit; from N = 11 up
+// 2^13-entry tiles are faster).  A smaller tile setting is kept.
 int dm_tile_bits_for(int prec, int nbits) {
   int T = tile_bits_for(prec, nbits);
   if (prec == BT_C64 && nbits <= 20) T = std::min(T, 12);
@@ -632,7 +641,7 @@ int bt_run_circuit(bt_state s, bt_circuit c, uint64_t seed, uint64_t traj, const
   std::vector<unsigned char> tables;
   std::vector<uint8_t> narrow;
   size_t tabsz = 0;
-  rc = static_plan(s, c, c->h, &c->sv_plan, T, &tables, &narrow, &tabsz);
+  rc = static_plan(s, c, false, T, &tables, &narrow, &tabsz);
   if (rc) return rc;
   if (nout) *nout = 0;
   return run_static(s, tables, narrow, tabsz, T, stats);
@@ -644,15 +653,11 @@ int bt_run_circuit_dm(bt_state rho, bt_circuit c, bt_run_stats* stats) {
   if (rc) return rc;
   if (rho->kind != BT_DM) return fail(BT_ERR_ARG, "bt_run_circuit_dm needs a density matrix");
   if (rho->n != c->h.n) return fail(BT_ERR_ARG, "circuit and state qubit counts differ");
-  btc::CircuitHost dm;
-  std::string msg;
-  rc = lower_dm(c->h, &dm, &msg);
-  if (rc) return fail(rc, msg);
   const int T = dm_tile_bits(rho);
   std::vector<unsigned char> tables;
   std::vector<uint8_t> narrow;
   size_t tabsz = 0;
-  rc = static_plan(rho, c, dm, &c->dm_plan, T, &tables, &narrow, &tabsz);
+  rc = static_plan(rho, c, true, T, &tables, &narrow, &tabsz);
   if (rc) return rc;
   return run_static(rho, tables, narrow, tabsz, T, stats);
 }
