@@ -88,6 +88,41 @@ __global__ void __launch_bounds__(256) k_apply1_low(float4* __restrict__ s, uint
   }
 }
 
+// Uncontrolled two-qubit operator on low target bits (both <= 6) of a
+// complex64 state: a wavefront reads 64 consecutive 16-byte pairs of
+// amplitudes (a 128-amplitude segment that holds whole groups) into LDS,
+// every lane forms its two outputs from the segment, and writes them back
+// with one 16-byte access, so reads and writes stay 1 KiB contiguous per
+// wavefront whatever the targets.  The workgroup walks 256-pair blocks in a
+// uniform trip count (nvec is a multiple of 256).
+__global__ void __launch_bounds__(256) k_apply2_low(float4* __restrict__ s, uint64_t nvec, ApplyArgs<float, 2> a) {
+  __shared__ float2 seg[4][128];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t mask = (uint32_t)a.off[3];
+  for (uint64_t b0 = (uint64_t)blockIdx.x * 256; b0 < nvec; b0 += (uint64_t)gridDim.x * 256) {
+    const float4 v = s[b0 + threadIdx.x];
+    seg[w][2 * lane] = make_float2(v.x, v.y);
+    seg[w][2 * lane + 1] = make_float2(v.z, v.w);
+    __syncthreads();
+    float2 y[2];
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const uint32_t i = 2u * (uint32_t)lane + (uint32_t)e;  // index in the segment
+      const uint32_t rest = i & ~mask;
+      int r = 0;
+#pragma unroll
+      for (int l = 1; l < 4; l++)
+        if ((i & mask) == (uint32_t)a.off[l]) r = l;
+      float2 acc = cmake<float2>(0, 0);
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc = cfma(acc, a.m[r + c * 4], seg[w][rest | (uint32_t)a.off[c]]);
+      y[e] = acc;
+    }
+    s[b0 + threadIdx.x] = make_float4(y[0].x * a.scale, y[0].y * a.scale, y[1].x * a.scale, y[1].y * a.scale);
+    __syncthreads();  // the segment is rewritten on the next trip
+  }
+}
+
 // Uncontrolled two-qubit operator on a complex64 state whose target bits
 // are both >= 1: every lane takes the two groups that differ in amplitude
 // bit 0 (consecutive amplitudes) with one 16-byte access per matrix index,
@@ -499,6 +534,13 @@ static int apply_t(bt_state_s* s, int nbits, const double* m, const Placement& p
     }
   }
   if constexpr (K == 2 && sizeof(R) == 4) {
+    // uncontrolled, both targets low: staged through LDS per wavefront
+    if (p.ctrl == 0 && p.nfixed == 2 && p.fixed[1] <= 6 && nbits >= 9) {
+      const uint64_t nvec = 1ull << (nbits - 1);  // 16-byte pairs, a multiple of 256
+      hipLaunchKernelGGL(k_apply2_low, dim3(grid_for(nvec)), dim3(256), 0, s->stream, (float4*)s->d, nvec, a);
+      BT_HIP(hipGetLastError());
+      return BT_OK;
+    }
     // uncontrolled, both targets in the middle bit range: two groups per lane
     if (p.ctrl == 0 && p.nfixed == 2 && p.fixed[0] >= 9 && p.fixed[1] <= 24) {
       hipLaunchKernelGGL(k_apply2_pair, dim3(grid_for(a.ngroups / 2)), dim3(256), 0, s->stream, a);

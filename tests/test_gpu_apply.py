@@ -126,10 +126,11 @@ def test_large_state_gate_roundtrip(bt, bit):
     assert abs(st.norm2() - 1) < 1e-5
 
 
-@pytest.mark.parametrize("bits", [(1, 2), (9, 10), (12, 20), (23, 24), (24, 25), (3, 22)])
+@pytest.mark.parametrize("bits", [(0, 1), (2, 5), (0, 6), (1, 2), (6, 7), (9, 10), (12, 20), (23, 24), (24, 25),
+                                  (3, 22)])
 def test_large_state_two_qubit_gates(bt, bits):
-    """N=26 complex64: a random two-qubit unitary on bit pairs on both sides
-    of the middle range (where the per-op kernel takes two groups per lane)
+    """N=26 complex64: a random two-qubit unitary on low bit pairs (staged
+    through LDS), on the middle range (two groups per lane) and around them
     matches a dense host application to 1e-6, and U then U^dagger restores
     the state."""
     N = 26
