@@ -56,11 +56,12 @@ HBM_PEAK_GBS = 8000.0
 # tiling's coalescing bits (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated,
 # tilings 13,3,16 and 13,4,16); a tiling not in the table reports no traffic
 TRAFFIC_RATIO = {3: 1.232, 4: 1.002}
-# k_round's FP32 FMA flops per algorithmic byte on this circuit at 3 coalescing
-# bits (rocprofv3 SQ_INSTS_VALU_FMA_F32 x 64 lanes x 2 over the same launches'
-# algorithmic bytes), and the non-packed FP32 vector peak it is measured against
-FMA_FLOP_PER_BYTE = {3: 11.52}
-VALU_FP32_PEAK_TFLOPS = 78.6
+# k_round's operator flops per algorithmic byte on this circuit at 3 coalescing
+# bits: the register operators run as v_mfma_f32_4x4x1f32 (512 flop each;
+# rocprofv3 SQ_INSTS_VALU_MFMA_MOPS_F32 x 512 over the same launches'
+# algorithmic bytes), against the FP32 matrix-core peak
+MFMA_FLOP_PER_BYTE = {3: 11.505}
+MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
@@ -633,7 +634,7 @@ def main():
     cb = (tuning or {}).get("coalesce_bits")
     unfused = unfused_bytes_per_trajectory(cfg)
     fused = nbytes / (args.ntraj * args.steps)
-    ratio, fpb = TRAFFIC_RATIO.get(cb), FMA_FLOP_PER_BYTE.get(cb)
+    ratio, fpb = TRAFFIC_RATIO.get(cb), MFMA_FLOP_PER_BYTE.get(cb)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -672,13 +673,13 @@ def main():
                                        "not measured for %s coalescing bits" % cb,
                      "launches": int(launches), "avg_launch_ms": round(kms / max(1, launches), 4),
                      "bytes_per_launch": round(nbytes / max(1, launches), 1),
-                     "valu": {"achieved_tflops": round(achieved * fpb / 1e3, 1) if fpb else None,
-                              "peak_tflops": VALU_FP32_PEAK_TFLOPS,
-                              "frac": round(achieved * fpb / 1e3 / VALU_FP32_PEAK_TFLOPS, 4) if fpb else None,
-                              "flop_per_byte": fpb,
-                              "note": "FP32 FMA work of the operators per algorithmic byte (rocprofv3 "
-                                      "SQ_INSTS_VALU_FMA_F32 x 64 lanes x 2, profiles/k_round_c3_n22_4096traj.txt) "
-                                      "against the non-packed FP32 vector peak (packed FP32 is not used)"}},
+                     "matrix": {"achieved_tflops": round(achieved * fpb / 1e3, 1) if fpb else None,
+                                "peak_tflops": MFMA_FP32_PEAK_TFLOPS,
+                                "frac": round(achieved * fpb / 1e3 / MFMA_FP32_PEAK_TFLOPS, 4) if fpb else None,
+                                "flop_per_byte": fpb,
+                                "note": "the register operators' FP32 matrix-core work per algorithmic byte "
+                                        "(rocprofv3 SQ_INSTS_VALU_MFMA_MOPS_F32 x 512, "
+                                        "profiles/k_round_c3_n22_4096traj.txt) against the FP32 MFMA peak"}},
     }
     out["precision"] = precision_leg(bt, c, args)
     log("precision", out["precision"])

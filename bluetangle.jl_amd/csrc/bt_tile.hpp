@@ -586,26 +586,110 @@ __host__ __device__ __forceinline__ void apply_reg(C<R>* v, const C<R>* mats) {
   }
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// The same operator on a complex64 tile, on the FP32 matrix cores.
+// v_mfma_f32_4x4x1f32 is 16 blocks of 4 x 4 x 1 products: lane l (block
+// b = l / 4) gets D[i] += A(lane 4b + i) * B(lane l), i = 0..3.  With B = one
+// real input of lane l's own group and A(lane 4b + i) = the operator's real
+// form at output row i of a 4-row slice, every lane computes 4 real outputs
+// of its own group from its own inputs: nothing moves between lanes, and the
+// 2^K x 2^K complex product (2^(K+1) real inputs) costs 2^(K+1) instructions
+// per 4 outputs, at twice the FP32 rate of the vector FMAs and beside them.
+// Output slice h holds rows 2h, 2h + 1 of the complex matrix as
+// (Re y_2h, Im y_2h, Re y_2h+1, Im y_2h+1); the lane with l % 4 = i holds
+// the slice's row 2h + (i >> 1), real part if i is even, else imaginary,
+// against the inputs (Re x_c, Im x_c): (Mr, -Mi) for a real part, (Mi, Mr)
+// for an imaginary one.
+template <int MASK>
+__device__ __forceinline__ void apply_reg_mfma(C<float>* v, const C<float>* mats) {
+  constexpr int K = __builtin_popcount(MASK);
+  static_assert(K == 1 || K == 2, "matrix-core register operators cover one and two bits");
+  constexpr int D = 1 << K, NS = D / 2;
+  constexpr int NR = 1 << kRB;
+  int P[K];
+  {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kRB; j++)
+      if ((MASK >> j) & 1) P[c++] = j;
+  }
+  const int i = threadIdx.x & 3;
+  const bool imag = (i & 1) != 0;
+  float a[NS][2 * D];
+#pragma unroll
+  for (int h = 0; h < NS; h++) {
+    const int r = 2 * h + (i >> 1);
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      const C<float> m = mats[r + c * D];
+      a[h][2 * c] = imag ? m.y : m.x;
+      a[h][2 * c + 1] = imag ? m.x : -m.y;
+    }
+  }
+#pragma unroll
+  for (int base = 0; base < NR; base++) {
+    if (base & MASK) continue;
+    C<float> x[D];
+    int idx[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+      idx[l] = base;
+#pragma unroll
+      for (int j = 0; j < K; j++)
+        if ((l >> j) & 1) idx[l] |= 1 << P[j];
+      x[l] = v[idx[l]];
+    }
+#pragma unroll
+    for (int h = 0; h < NS; h++) {
+      floatx4_t d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        d = __builtin_amdgcn_mfma_f32_4x4x1f32(a[h][2 * c], x[c].x, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_4x4x1f32(a[h][2 * c + 1], x[c].y, d, 0, 0, 0);
+      }
+      v[idx[2 * h]] = cmake<C<float>>(d[0], d[1]);
+      v[idx[2 * h + 1]] = cmake<C<float>>(d[2], d[3]);
+    }
+  }
+}
+#endif
+
+// Register operator with the mask a compile-time constant: complex64 on the
+// device runs on the matrix cores, everything else (complex128, the host
+// emulation) as vector FMAs.
+template <typename R, int MASK>
+__host__ __device__ __forceinline__ void apply_regk(C<R>* v, const C<R>* m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(R) == 4) {
+    apply_reg_mfma<MASK>(v, m);
+    return;
+  }
+#endif
+  apply_reg<R, MASK>(v, m);
+}
+
 template <typename R>
 __host__ __device__ __forceinline__ void apply_mask(int mask, C<R>* v, const C<R>* m) {
   // adjacent register pairs are the common case (brickwork pairs after the
   // phase planner): test them first, every level of the dispatch costs moves
   if (__builtin_expect(mask == 0x06, 1)) {
-    apply_reg<R, 0x06>(v, m);
+    apply_regk<R, 0x06>(v, m);
     return;
   }
   if constexpr (TileCfg<R>::RB == 5) {
     if (__builtin_expect(mask == 0x18, 1)) {
-      apply_reg<R, 0x18>(v, m);
+      apply_regk<R, 0x18>(v, m);
       return;
     }
   }
   switch (mask) {
 #define BT_CASE(M) \
-  case M: apply_reg<R, M>(v, m); break;
+  case M: apply_regk<R, M>(v, m); break;
 #define BT_CASE5(M)                                   \
   case M:                                             \
-    if constexpr (TileCfg<R>::RB == 5) apply_reg<R, M>(v, m); \
+    if constexpr (TileCfg<R>::RB == 5) apply_regk<R, M>(v, m); \
     break;
     BT_CASE(1) BT_CASE(2) BT_CASE(4) BT_CASE(8) BT_CASE(3) BT_CASE(5) BT_CASE(6) BT_CASE(9) BT_CASE(10) BT_CASE(12)
     BT_CASE5(16) BT_CASE5(17) BT_CASE5(18) BT_CASE5(20) BT_CASE5(24)
@@ -630,7 +714,7 @@ __host__ __device__ __forceinline__ void op_fence() {
 template <typename R>
 __host__ __device__ __forceinline__ void apply_template(int tm, C<R>* v, const C<R>* mats, const ROp* ops) {
 #define BT_A(M, I)                              \
-  apply_reg<R, M>(v, mats + ops[I].moff); \
+  apply_regk<R, M>(v, mats + ops[I].moff); \
   op_fence();
   if (tm == 1) {
     BT_A(0x0c, 0)
