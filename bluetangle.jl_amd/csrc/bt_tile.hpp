@@ -1028,12 +1028,32 @@ __host__ __device__ __forceinline__ void opaque(T& x) {
 #endif
 }
 
+// The same for a value that is equal in every lane, kept in a scalar
+// register (moved there first: a lane-uniform value the compiler cannot
+// prove uniform would otherwise occupy a vector register per lane).
+template <typename T>
+__host__ __device__ __forceinline__ void opaque_uniform(T& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)x;
+    x = (T)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(u >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
+  } else {
+    x = (T)__builtin_amdgcn_readfirstlane((int)x);
+  }
+  asm volatile("" : "+s"(x));
+#else
+  (void)x;
+#endif
+}
+
+// gin / gout are the same in every lane (pass geometry); gthr_* are per thread.
 template <typename O>
 __host__ __device__ __forceinline__ void opaque_io(ThreadIO<O>& io) {
 #pragma unroll
   for (int j = 0; j < kRB; j++) {
-    opaque(io.gin[j]);
-    opaque(io.gout[j]);
+    opaque_uniform(io.gin[j]);
+    opaque_uniform(io.gout[j]);
   }
   opaque(io.gthr_in);
   opaque(io.gthr_out);
@@ -1074,6 +1094,14 @@ __host__ __device__ __forceinline__ bool uflag(bool x) {
   return __builtin_amdgcn_readfirstlane((int)x) != 0;
 #else
   return x;
+#endif
+}
+
+// Wait until every vector-memory access this wave issued has completed
+// (s_waitcnt vmcnt(0), the LDS and export counters left alone).
+__host__ __device__ __forceinline__ void vm_wait_all() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_s_waitcnt(0x0F70);
 #endif
 }
 
@@ -1266,13 +1294,21 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
   // the next tile's base is a masked add: ((base | ~M) + deposit(tstep)) & M
   const uint64_t M = uniform_u64(tile_base(g, ~0ull)), S = uniform_u64(tile_base(g, tstep));
   uint64_t base = uniform_u64(tile_base(g, t0));
-  if (!fresh && t0 < tend) fetch_regs<R, O>(nx, state, io, base);
+  // Software pipeline over the tiles (disjoint): tile t's operators run while
+  // the loads of tile t+1 and the stores of tile t-1 are in flight.  The wait
+  // for tile t+1's loads is placed before tile t's stores are issued: the
+  // vector-memory counter does not order loads against stores, so a wait for
+  // loads issued before stores also waits for the stores; waiting first lets
+  // tile t's stores complete under tile t+1's operators instead.
+  if (t0 < tend) {
+    if (!fresh) fetch_regs<R, O>(nx, state, io, base);
+    take_regs<R, O>(v, nx, io, base, scale, fresh);
+    if (!fresh && t0 + tstep < tend) fetch_regs<R, O>(nx, state, io, ((base | ~M) + S) & M);
+  }
   for (uint64_t t = t0; t < tend; t += tstep) {
     opaque_io(io);
     const uint64_t nbase = ((base | ~M) + S) & M;
-    take_regs<R, O>(v, nx, io, base, scale, fresh);
-    // the next tile's loads fly while this one computes (tiles are disjoint)
-    if (!fresh && t + tstep < tend) fetch_regs<R, O>(nx, state, io, nbase);
+    const bool more = t + tstep < tend;
     int8_t cur[kRB];
 #pragma unroll
     for (int j = 0; j < kRB; j++) cur[j] = g.io_in[j];
@@ -1331,6 +1367,20 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
     double acc[17];
 #pragma unroll
     for (int j = 0; j < 17; j++) acc[j] = 0.0;
+    {
+      // tile t+1's loads (and tile t-1's stores) are done; prefetched values
+      // the register allocator parked in scratch come back here, before the
+      // stores, so that reloading them does not wait for the stores.  On
+      // every path (also the last tile, or when nothing was fetched): the
+      // compiler's wait bookkeeping merges paths, and one without this wait
+      // would make it wait at the take below, after the stores.
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        opaque(nx[r].x);
+        opaque(nx[r].y);
+      }
+      vm_wait_all();
+    }
     store_regs<R, O>(v, state, g, io, base, acc);
     if (g.red == bti::RED_HIST) {
       switch (g.red_k) {  // norm + 2^red_k bins
@@ -1341,6 +1391,10 @@ __device__ __forceinline__ void tile_pass(C<R>* __restrict__ state, const PassGe
       }
     } else if (g.need_norm) {
       wave_accumulate<1>(acc, wred);
+    }
+    if (more) {  // tile t+1 into registers, tile t+2's loads in flight
+      take_regs<R, O>(v, nx, io, nbase, scale, fresh);
+      if (!fresh && t + 2 * tstep < tend) fetch_regs<R, O>(nx, state, io, ((nbase | ~M) + S) & M);
     }
     base = nbase;
   }

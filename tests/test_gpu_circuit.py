@@ -286,3 +286,43 @@ def test_rounds_with_partial_workgroups(bt, tpb):
             assert np.abs(states[tr] - r).max() < 1e-5
     finally:
         _lib.check(_lib.lib().bt_set_tuning(0, 0, 0, 0))
+
+
+def _haar(rng, d):
+    q, r = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+@pytest.mark.parametrize("N", [13, 16])
+def test_tile_pass_every_pair(bt, N):
+    """A random unitary on every qubit and on every ordered qubit pair, plus a
+    controlled one per qubit, noiseless, through bt_run_circuit: fused
+    register-tile passes whose register operators land on many register
+    masks (complex64: the matrix-core form; complex128: vector FMAs),
+    against the oracle's application of the same matrices."""
+    from bluetangle_jl_amd.circuit import Circuit
+    rng = np.random.default_rng(1000 + N)
+    ops_bt, ops_ref = [], []
+    for q in range(1, N + 1):
+        u = _haar(rng, 2)
+        ops_bt.append(bt.Op("U1", u, q))
+        ops_ref.append(O.ref_gate("U1", q, mat=u))
+    for a in range(1, N + 1):
+        for b in range(a + 1, N + 1):
+            u = _haar(rng, 4)
+            x, y = (a, b) if rng.random() < 0.5 else (b, a)
+            ops_bt.append(bt.Op("U2", u, x, y))
+            ops_ref.append(O.ref_gate("U2", x, y, mat=u))
+    for q in range(1, N + 1):
+        u = _haar(rng, 2)
+        c = 1 + (q + N // 2) % N
+        ops_bt.append(bt.Op("UC", u, q, control=c))
+        ops_ref.append(O.ref_gate("UC", q, control=c, mat=u))
+    r, _, _, _, _ = O.run_sv(N, ops_ref, None, O.uniforms(0, 0, 16))
+    circ = Circuit(ops_bt, N, None)
+    for prec in ("c64", "c128"):
+        st = bt.zero_state(N, prec)
+        outs, stats = circ.run_on(st, seed=0, traj=0)
+        s = st.to_numpy()
+        err = float(np.abs(s - r).max())
+        assert err <= TOL[prec], (prec, N, err)
