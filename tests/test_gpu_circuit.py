@@ -297,9 +297,10 @@ def _haar(rng, d):
 def test_tile_pass_every_pair(bt, N):
     """A random unitary on every qubit and on every ordered qubit pair, plus a
     controlled one per qubit, noiseless, through bt_run_circuit: fused
-    register-tile passes whose register operators land on many register
-    masks (complex64: the matrix-core form; complex128: vector FMAs),
-    against the oracle's application of the same matrices."""
+    register-tile passes (complex64: the matrix-core form; complex128: vector
+    FMAs) against the oracle's application of the same matrices.  At N=13 the
+    passes use all 15 complex64 register masks (5 one-bit, 10 two-bit), as
+    the host emulation's dump of the same circuit's pass geometries shows."""
     from bluetangle_jl_amd.circuit import Circuit
     rng = np.random.default_rng(1000 + N)
     ops_bt, ops_ref = [], []
