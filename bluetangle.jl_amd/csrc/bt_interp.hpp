@@ -392,6 +392,25 @@ struct Interp {
     return c;
   }
 
+  // Greedy over the queue, seeded with the first operator: the target bits
+  // of the operators that fit in `cap` bits, in queue order, skipping those
+  // that touch a bit an earlier skipped operator touches.
+  BT_HD uint64_t greedy_bits(int cap) {
+    uint64_t g = 0, blocked = 0;
+    for (int i = 0; i < nq; i++) {
+      uint64_t b = maskof(Q(i).bits, Q(i).k);
+      uint64_t touch = b | Q(i).ctrl;
+      if (touch & blocked) {
+        blocked |= touch;
+        continue;
+      }
+      // the first operator always seeds the tile, even past the coalescing reserve
+      if (popc(g | b) <= cap || (g == 0 && popc(b) <= T)) g |= b;
+      else blocked |= touch;
+    }
+    return g;
+  }
+
   BT_HD void make_plan(Plan* plan) {
     uint64_t need = 0;  // reduction bits that must be inside the tile
     if (red_req == RED_RHO && !red_done)
@@ -403,22 +422,24 @@ struct Interp {
       tile = fill_low(need);
     } else {
       // candidate 1: greedy over the queue, seeded with the first operator
-      uint64_t g = 0, blocked = 0;
-      const int cap = T - cmin;
-      for (int i = 0; i < nq; i++) {
-        uint64_t b = maskof(Q(i).bits, Q(i).k);
-        uint64_t touch = b | Q(i).ctrl;
-        if (touch & blocked) {
-          blocked |= touch;
-          continue;
-        }
-        // the first operator always seeds the tile, even past the coalescing reserve
-        if (popc(g | b) <= cap || (g == 0 && popc(b) <= T)) g |= b;
-        else blocked |= touch;
-      }
-      uint64_t t1 = fill_low(g);
+      uint64_t t1 = fill_low(greedy_bits(T - cmin));
       int32_t s1[kPMax];
       int n1 = select_ops(t1, s1);
+      // candidate 1b: the same with one more low bit reserved (twice the
+      // contiguous run per access, 128 B for complex64 at 3 reserved bits),
+      // taken when it selects as many operators
+      if (cmin + 1 < T) {
+        const uint64_t t1b = fill_low(greedy_bits(T - cmin - 1));
+        if (t1b != t1) {
+          int32_t s1b[kPMax];
+          const int n1b = select_ops(t1b, s1b);
+          if (n1b >= n1) {
+            t1 = t1b;
+            n1 = n1b;
+            for (int i = 0; i < n1; i++) s1[i] = s1b[i];
+          }
+        }
+      }
       // candidate 2: the low tile [0, T)
       uint64_t t2 = fill_low(0);
       int32_t s2[kPMax];
