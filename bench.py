@@ -55,7 +55,7 @@ HBM_PEAK_GBS = 8000.0
 # k_round's measured HBM traffic / algorithmic bytes on this circuit, by the
 # tiling's coalescing bits (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated,
 # tilings 13,3,16 and 13,4,16); a tiling not in the table reports no traffic
-TRAFFIC_RATIO = {3: 1.232, 4: 1.002}
+TRAFFIC_RATIO = {3: 1.125, 4: 1.000}
 # k_round's operator flops per algorithmic byte on this circuit at 3 coalescing
 # bits: the register operators run as v_mfma_f32_4x4x1f32 (512 flop each;
 # rocprofv3 SQ_INSTS_VALU_MFMA_MOPS_F32 x 512 over the same launches'
