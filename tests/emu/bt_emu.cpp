@@ -69,6 +69,10 @@ static int64_t g_gstat[9];
 
 static FILE* g_geom_dump = nullptr;
 static int64_t g_tmpl_phases = 0;  // phases run through a straight-line template
+// passes whose tile holds local bit `g_deep_bit` (the bit right after the
+// coalescing reserve: twice the contiguous run per access)
+static int g_deep_bit = -1;
+static int64_t g_deep_passes = 0;
 
 extern "C" void emu_set_exec(int mode) { g_exec = mode; }
 // Append every register-tile pass geometry (PassGeom, then nmats complex
@@ -81,6 +85,12 @@ extern "C" int emu_dump_geometry(const char* path) {
 extern "C" int64_t emu_template_phases(int reset) {
   const int64_t v = g_tmpl_phases;
   if (reset) g_tmpl_phases = 0;
+  return v;
+}
+
+extern "C" int64_t emu_deep_coalesced_passes(int reset) {
+  const int64_t v = g_deep_passes;
+  if (reset) g_deep_passes = 0;
   return v;
 }
 
@@ -111,6 +121,8 @@ static int tile_pass_emu(std::vector<bt::C<R>>& st, int nbits, const bti::Plan& 
   g_gstat[1] += g.nops;
   for (int o = 0; o < g.nops; o++) g_gstat[1 + g.ops[o].k]++;
   g_gstat[6] += g.nph;
+  for (int i = 0; i < g.T; i++)
+    if (g.tile[i] == g_deep_bit) g_deep_passes++;
   const int T = g.T;
   const R scale = (R)g.scale;
   const uint64_t ntiles = 1ull << (nbits - T);
@@ -347,6 +359,7 @@ extern "C" int emu_run(int nbits, const bt_ir_op* ops, int nops, int ntop, const
   btc::CircuitHost c;
   int rc = btc::analyze(nbits, ops, nops, ntop, pool, npool, &c, &g_msg);
   if (rc) return rc;
+  g_deep_bit = cmin;
   return run_host(c, nbits, seed, traj, T, cmin, fold, state, outs, outcap, nout, stats, tol_exact);
 }
 

@@ -367,6 +367,37 @@ def test_register_tile_pass_brickwork(emu, bt, mode):
     assert emu.emu_template_phases(1) > 0
 
 
+def test_planner_reserves_a_deeper_run_when_free(emu, bt):
+    """The pass planner reserves one low tile bit beyond the configured
+    coalescing bits whenever the greedy pick still selects as many
+    operators (128-byte runs for complex64 at 3 bits): on the benchmark's
+    noisy brickwork at its size (N=22; 4 of its layers) at least 3/4 of the
+    passes hold bit 3 (8 of 10; a reserve of 3 bits alone gives 6 of 10),
+    without more passes than a plain 4-bit reserve needs, and the state
+    still matches the oracle."""
+    N = 22
+    spec = brickwork_spec(N, 4, 3, ad_gamma=5e-3, mz_prob=0.1)
+    nm_args = ("depolarizing", 1e-3, "depolarizing", 1e-2)
+    r, routs, used, bnd = oracle_run(spec, N, nm_args, 2, tol=1e-6)
+    assert not bnd
+    emu.emu_deep_coalesced_passes.restype = ctypes.c_int64
+    res = {}
+    emu.emu_set_exec(2)  # the complex64 register-tile pass, as the device runs it
+    try:
+        for cmin in (3, 4):
+            emu.emu_deep_coalesced_passes(1)
+            s, outs, stats = emu_run(emu, bt, spec, N, bt.NoiseModel(*nm_args), 2, T=13, cmin=cmin, tol=1e-6)
+            res[cmin] = (int(stats[0]), int(emu.emu_deep_coalesced_passes(1)))
+            assert outs == routs
+            assert np.abs(s - r).max() < 1e-5
+    finally:
+        emu.emu_set_exec(0)
+    print("passes, passes holding the bit after the reserve:", res)
+    passes3, deep3 = res[3]
+    assert 4 * deep3 >= 3 * passes3, res
+    assert passes3 <= res[4][0], res
+
+
 def _apply_block_dense(state, N, k, bits, m):
     """psi <- M psi on bits (bits[0] = MSB of M's index), plain numpy."""
     psi = state.copy()
