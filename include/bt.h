@@ -263,12 +263,13 @@ int64_t bt_graph_passes(bt_graph g);
 int bt_graph_destroy(bt_graph g);
 
 /* Tile width (bits held in LDS per pass) of the circuit executors for a
- * precision; 0 restores the default (14 for BT_C64, 13 for BT_C128).  Set by
+ * precision; 0 restores the default (13 for BT_C64, 12 for BT_C128).  Set by
  * the autotuner.  Process-wide. */
 int bt_set_tile_bits(int prec, int tile_bits);
 
 /* Tuning of the circuit executors (process-wide, per precision): tile bits,
- * low global bits every tile keeps for coalescing, tiles per workgroup of the
+ * low global bits every tile keeps for coalescing (a pass keeps one more when
+ * its operators still fit), tiles per workgroup of the
  * rounds executor; 0 restores a default. */
 typedef struct {
   int32_t tile_bits;
